@@ -1,0 +1,289 @@
+// fft_core.h -- register/LDS Stockham FFT building blocks for gfx950 (wave64).
+//
+// A line of N complex points is held by TPL = N/E threads, E points per thread.
+// A transform is a sequence of Stockham passes with radices R_p (each R_p | E).
+// Pass p, after L = R_0...R_{p-1} points are combined, m = N/R_p:
+//     a_t = x[j + t*m] * w^(t*k),  k = j mod L,  w = exp(sign*2*pi*i/(L*R_p))
+//     y[(j/L)*L*R_p + k + s*L] = sum_t a_t * exp(sign*2*pi*i*s*t/R_p)
+// Thread `tid` owns butterflies j = tid + TPL*q (q < E/R_p), so its registers
+// hold v[q*R_p + t].  Before the first pass (and after the last one) thread
+// `tid` holds points  tid + TPL*q + t*(N/R)  -- the "boundary layout" of the
+// boundary radix R.  A transform whose first radix equals the previous
+// transform's last radix can therefore start from registers: FFT -> pointwise
+// op -> IFFT chains never touch LDS at the boundary.  Inner passes exchange
+// through LDS.  The radix sequence is walked forwards (ORD_FWD) or backwards
+// (ORD_REV); chained transforms alternate the order.
+//
+// This replaces the cuFFT Z2Z plans of GPUE (src/init.cu:309-319,
+// src/ds.cu:11-79): unnormalised, sign -1 forward, +1 inverse.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gpue {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return mk(a.x * s, a.y * s); }
+// multiply by sign*i
+template <typename T, int SIGN>
+__device__ __forceinline__ cx<T> mul_si(cx<T> a) {
+  if constexpr (SIGN < 0) return mk(a.y, -a.x);
+  else return mk(-a.y, a.x);
+}
+
+// cos/sin(2*pi*m/48) for m = 0..47 -- covers every radix used here (2,3,4,8,12,16)
+struct Tw48 {
+  static constexpr double c[48] = {
+      1.0, 0.99144486137381041114, 0.96592582628906828675, 0.92387953251128675613,
+      0.86602540378443864676, 0.79335334029123516458, 0.70710678118654752440, 0.60876142900872063942,
+      0.5, 0.38268343236508977173, 0.25881904510252076235, 0.13052619222005159155,
+      0.0, -0.13052619222005159155, -0.25881904510252076235, -0.38268343236508977173,
+      -0.5, -0.60876142900872063942, -0.70710678118654752440, -0.79335334029123516458,
+      -0.86602540378443864676, -0.92387953251128675613, -0.96592582628906828675, -0.99144486137381041114,
+      -1.0, -0.99144486137381041114, -0.96592582628906828675, -0.92387953251128675613,
+      -0.86602540378443864676, -0.79335334029123516458, -0.70710678118654752440, -0.60876142900872063942,
+      -0.5, -0.38268343236508977173, -0.25881904510252076235, -0.13052619222005159155,
+      0.0, 0.13052619222005159155, 0.25881904510252076235, 0.38268343236508977173,
+      0.5, 0.60876142900872063942, 0.70710678118654752440, 0.79335334029123516458,
+      0.86602540378443864676, 0.92387953251128675613, 0.96592582628906828675, 0.99144486137381041114};
+  static __device__ __forceinline__ constexpr double cosv(int m) { return c[((m % 48) + 48) % 48]; }
+  static __device__ __forceinline__ constexpr double sinv(int m) { return c[((m - 12) % 48 + 48) % 48]; }
+};
+
+// multiply a by exp(SIGN*2*pi*i*m/R), with the exact special cases folded
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ cx<T> twc(cx<T> a, int m) {
+  m = ((m % R) + R) % R;
+  if (m == 0) return a;
+  if (2 * m == R) return mk(-a.x, -a.y);
+  if (4 * m == R) return mul_si<T, SIGN>(a);
+  if (4 * m == 3 * R) return mul_si<T, -SIGN>(a);
+  const int m48 = m * (48 / R);
+  const T c = (T)Tw48::cosv(m48);
+  const T s = (T)(SIGN * Tw48::sinv(m48));
+  return mk(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+// In-place DFT of R points in registers, natural order in and out.
+template <typename T, int R, int SIGN>
+struct DFT;
+
+template <typename T, int SIGN>
+struct DFT<T, 2, SIGN> {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    cx<T> t0 = a[0] + a[1], t1 = a[0] - a[1];
+    a[0] = t0; a[1] = t1;
+  }
+};
+
+template <typename T, int SIGN>
+struct DFT<T, 3, SIGN> {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    const T c = (T)-0.5;
+    const T s = (T)(SIGN * 0.86602540378443864676);
+    cx<T> sum = a[1] + a[2], dif = a[1] - a[2];
+    cx<T> b0 = a[0] + sum;
+    cx<T> m = mk(a[0].x + c * sum.x, a[0].y + c * sum.y);
+    cx<T> r = mk(-s * dif.y, s * dif.x);   // i*s*dif
+    a[0] = b0; a[1] = m + r; a[2] = m - r;
+  }
+};
+
+template <typename T, int SIGN>
+struct DFT<T, 4, SIGN> {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    cx<T> s0 = a[0] + a[2], s1 = a[0] - a[2];
+    cx<T> s2 = a[1] + a[3], s3 = mul_si<T, SIGN>(a[1] - a[3]);
+    a[0] = s0 + s2; a[2] = s0 - s2; a[1] = s1 + s3; a[3] = s1 - s3;
+  }
+};
+
+// R = P*Q:  t = t1 + P*t2,  s = Q*s1 + s2
+//   b[Q s1 + s2] = sum_t1 wP^(s1 t1) * w^(s2 t1) * sum_t2 a[t1 + P t2] * wQ^(s2 t2)
+template <typename T, int P, int Q, int SIGN>
+__device__ __forceinline__ void dft_compose(cx<T>* a) {
+  constexpr int R = P * Q;
+  cx<T> c[P][Q];
+#pragma unroll
+  for (int t1 = 0; t1 < P; ++t1) {
+    cx<T> col[Q];
+#pragma unroll
+    for (int t2 = 0; t2 < Q; ++t2) col[t2] = a[t1 + P * t2];
+    DFT<T, Q, SIGN>::run(col);
+#pragma unroll
+    for (int s2 = 0; s2 < Q; ++s2) c[t1][s2] = twc<T, R, SIGN>(col[s2], s2 * t1);
+  }
+#pragma unroll
+  for (int s2 = 0; s2 < Q; ++s2) {
+    cx<T> row[P];
+#pragma unroll
+    for (int t1 = 0; t1 < P; ++t1) row[t1] = c[t1][s2];
+    DFT<T, P, SIGN>::run(row);
+#pragma unroll
+    for (int s1 = 0; s1 < P; ++s1) a[Q * s1 + s2] = row[s1];
+  }
+}
+
+template <typename T, int SIGN>
+struct DFT<T, 8, SIGN> {
+  static __device__ __forceinline__ void run(cx<T>* a) { dft_compose<T, 4, 2, SIGN>(a); }
+};
+template <typename T, int SIGN>
+struct DFT<T, 12, SIGN> {
+  static __device__ __forceinline__ void run(cx<T>* a) { dft_compose<T, 4, 3, SIGN>(a); }
+};
+template <typename T, int SIGN>
+struct DFT<T, 16, SIGN> {
+  static __device__ __forceinline__ void run(cx<T>* a) { dft_compose<T, 4, 4, SIGN>(a); }
+};
+
+// ---------------------------------------------------------------------------
+// Plans: E points per thread and the radix sequence for each supported N.
+// ---------------------------------------------------------------------------
+template <int N>
+struct Plan;
+#define GPUE_PLAN(N_, E_, NP_, ...)                                     \
+  template <>                                                           \
+  struct Plan<N_> {                                                     \
+    static constexpr int N = N_, E = E_, TPL = N_ / E_, NP = NP_;       \
+    static constexpr int R[NP_] = {__VA_ARGS__};                        \
+  };
+GPUE_PLAN(16, 16, 1, 16)
+GPUE_PLAN(32, 16, 2, 16, 2)
+GPUE_PLAN(64, 16, 2, 16, 4)
+GPUE_PLAN(128, 16, 2, 16, 8)
+GPUE_PLAN(192, 12, 3, 12, 4, 4)
+GPUE_PLAN(256, 16, 2, 16, 16)
+GPUE_PLAN(384, 24, 3, 12, 8, 4)
+GPUE_PLAN(512, 16, 3, 16, 16, 2)
+GPUE_PLAN(1024, 16, 3, 16, 16, 4)
+#undef GPUE_PLAN
+
+enum { ORD_FWD = 0, ORD_REV = 1 };
+
+template <int N, int ORD>
+struct Seq {
+  using P = Plan<N>;
+  static constexpr int radix(int p) { return ORD == ORD_FWD ? P::R[p] : P::R[P::NP - 1 - p]; }
+  static constexpr int first() { return radix(0); }
+  static constexpr int last() { return radix(P::NP - 1); }
+  // L before pass p
+  static constexpr int Lbefore(int p) {
+    int L = 1;
+    for (int q = 0; q < p; ++q) L *= radix(q);
+    return L;
+  }
+};
+
+// Point index held in register slot r of thread tid in the boundary layout of radix R.
+template <int N, int R>
+__device__ __forceinline__ constexpr int bpos(int tid, int r) {
+  constexpr int TPL = Plan<N>::TPL;
+  return tid + TPL * (r / R) + (r % R) * (N / R);
+}
+
+// Twiddle table: tw[m] = exp(-2*pi*i*m/N) (SIGN>0 uses the conjugate).
+template <typename T, int SIGN>
+__device__ __forceinline__ cx<T> twl(const cx<T>* __restrict__ tw, int m) {
+  cx<T> w = tw[m];
+  if constexpr (SIGN > 0) w.y = -w.y;
+  return w;
+}
+
+// Compute pass p (radix R, L) on registers v[E] of thread tid.
+template <typename T, int N, int R, int L, int SIGN>
+__device__ __forceinline__ void pass_compute(cx<T>* v, int tid, const cx<T>* __restrict__ tw) {
+  constexpr int E = Plan<N>::E, TPL = Plan<N>::TPL, NB = E / R;
+  constexpr int STEP = N / (L * R);
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    cx<T>* a = v + q * R;
+    if constexpr (L > 1) {
+      const int j = tid + TPL * q;
+      const int k = j % L;
+#pragma unroll
+      for (int t = 1; t < R; ++t) a[t] = cmul(a[t], twl<T, SIGN>(tw, t * k * STEP));
+    }
+    DFT<T, R, SIGN>::run(a);
+  }
+}
+
+// Output position of register slot (q, s) after pass (R, L).
+template <int N, int R, int L>
+__device__ __forceinline__ int pass_out(int tid, int q, int s) {
+  constexpr int TPL = Plan<N>::TPL;
+  const int j = tid + TPL * q;
+  const int k = j % L;
+  return (j / L) * L * R + k + s * L;
+}
+
+// LDS addressing: element e of a line.  Row lines: base + pad(e); strided
+// lines: (e * LS + c) where LS = lines per tile (lanes vary in c fastest).
+struct LdsRow {
+  int base;
+  __device__ __forceinline__ int at(int e) const { return base + e + (e >> 4); }
+  static constexpr int footprint(int N) { return N + N / 16; }
+};
+struct LdsCol {
+  int c, LS;
+  __device__ __forceinline__ int at(int e) const { return e * LS + c; }
+};
+
+// Full transform on registers: input in boundary layout of first radix,
+// output in boundary layout of last radix.  Inner passes exchange via LDS.
+// `active` predicates LDS traffic for padding threads (they still hit barriers).
+template <typename T, int N, int ORD, int SIGN, typename LA>
+__device__ __forceinline__ void fft_regs(cx<T>* v, int tid, const cx<T>* __restrict__ tw, cx<T>* lds,
+                                         const LA& la, bool active) {
+  using S = Seq<N, ORD>;
+  constexpr int NP = Plan<N>::NP;
+  constexpr int E = Plan<N>::E;
+  // pass 0
+  pass_compute<T, N, S::radix(0), 1, SIGN>(v, tid, tw);
+  if constexpr (NP > 1) {
+    // exchange 0 -> 1
+    __syncthreads();  // previous users of LDS are done
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < E; ++r)
+        lds[la.at(pass_out<N, S::radix(0), 1>(tid, r / S::radix(0), r % S::radix(0)))] = v[r];
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) v[r] = lds[la.at(bpos<N, S::radix(1)>(tid, r))];
+    }
+    pass_compute<T, N, S::radix(1), S::Lbefore(1), SIGN>(v, tid, tw);
+  }
+  if constexpr (NP > 2) {
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < E; ++r)
+        lds[la.at(pass_out<N, S::radix(1), S::Lbefore(1)>(tid, r / S::radix(1), r % S::radix(1)))] = v[r];
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) v[r] = lds[la.at(bpos<N, S::radix(2)>(tid, r))];
+    }
+    pass_compute<T, N, S::radix(2), S::Lbefore(2), SIGN>(v, tid, tw);
+  }
+  static_assert(NP <= 3, "extend fft_regs for more passes");
+}
+
+}  // namespace gpue

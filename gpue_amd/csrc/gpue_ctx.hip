@@ -1,0 +1,753 @@
+// gpue_ctx.hip -- the C ABI (include/gpue_amd.h): per-problem context, the
+// step schedule of fused sweeps, energy, renormalisation and vortex detection.
+//
+// Host control flow restates GPUE's evolve() (src/evolution.cu:260-908) with
+// the host I/O left to the caller; device work is queued on one HIP stream
+// per context and nothing is allocated inside the step loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "eqn.h"
+#include "gpue_amd.h"
+#include "misc_kernels.h"
+
+namespace gpue {
+void make_grids(int dimnum, int xDim, int yDim, int zDim, double xMax, double yMax, double zMax,
+                std::vector<double>* g, double* d);
+void vortex_mask(const double* x, int xDim, double radius, std::vector<unsigned char>& m);
+void twiddles(int n, std::vector<double>& re, std::vector<double>& im);
+}  // namespace gpue
+
+using namespace gpue;
+
+namespace {
+
+struct Sweep {
+  bool row;
+  int axis;
+  int mode;
+  unsigned flags;
+  double invN;
+};
+
+constexpr int TIMING_PAIRS = 256;
+
+}  // namespace
+
+struct gpue_ctx {
+  gpue_params p;
+  int prec = 64;
+  size_t esz = 16;   // bytes per complex element
+  long N = 0;
+  int dims[3] = {1, 1, 1};
+  double d[3] = {1, 1, 1};
+  std::vector<double> grid[6];
+  hipStream_t stream = nullptr;
+  void* psi = nullptr;
+  void* tmp = nullptr;
+  double* dgrid[6] = {};
+  std::map<int, void*> tw;   // N -> twiddle table in the working precision
+  double* partials = nullptr;
+  size_t partials_n = 0;
+  unsigned* counter = nullptr;
+  double* results = nullptr;   // [0] norm  [1] kinetic  [2..5] position terms
+  int v_kind = V_HARMONIC, a_kind = A_ROTATION, k_kind = K_STANDARD;
+  double* Vt = nullptr;
+  double* Kt = nullptr;
+  double* At[3] = {};
+  bool az_zero = true;
+  EqnProg* veq = nullptr;
+  bool pending_norm = false;
+  double last_energy = 0.0;
+  std::string err;
+  // vortex scratch
+  unsigned char* vmask = nullptr;
+  double vmask_radius = -1.0;
+  int* vcand = nullptr;
+  int* vmarker = nullptr;
+  int* vfound = nullptr;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_kind;
+  int ev_used = 0;
+  std::vector<std::string> kind_names;
+  std::vector<long> kind_n;
+  std::vector<double> kind_ms;
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static gpue_status fail(gpue_ctx* c, gpue_status s, const std::string& msg) {
+  if (c) c->err = msg;
+  return s;
+}
+
+#define HIPCHK(c, expr)                                                                      \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return fail((c), GPUE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+static double* dev_copy(const std::vector<double>& v, hipError_t* e) {
+  double* p = nullptr;
+  *e = hipMalloc((void**)&p, std::max<size_t>(1, v.size()) * sizeof(double));
+  if (*e != hipSuccess) return nullptr;
+  *e = hipMemcpy(p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
+  return p;
+}
+
+static Phys make_phys(const gpue_ctx* c, int gstate, double time) {
+  const gpue_params& p = c->p;
+  Phys f;
+  std::memset(&f, 0, sizeof(f));
+  f.xDim = c->dims[0];
+  f.yDim = c->dims[1];
+  f.zDim = c->dims[2];
+  f.dimnum = p.dimnum;
+  f.gstate = gstate;
+  f.gpe = p.gpe;
+  f.v_kind = c->v_kind;
+  f.a_kind = c->a_kind;
+  f.k_kind = c->k_kind;
+  const double Dt = gstate == 0 ? p.gdt : p.dt;   // src/evolution.cu:344-351
+  f.cV = Dt / (2 * p.hbar);
+  f.cH = (0.5 * Dt) / p.hbar;
+  f.g = p.g;
+  f.cK = Dt / p.hbar;
+  f.Dt = Dt;
+  f.hb2m = p.hbar * p.hbar / (2 * p.mass);
+  f.mass = p.mass;
+  f.omega = p.omega;
+  f.omegaX = p.omegaX;
+  f.omegaY = p.omegaY;
+  f.omegaZe = p.dimnum == 3 ? p.omegaZ : 0.0;   // items[5] (src/operators.cu:765-767)
+  f.gammaY = p.gammaY;
+  f.x0 = p.x0_shift;
+  f.y0 = p.y0_shift;
+  f.z0e = p.dimnum == 3 ? p.z0_shift : 0.0;     // items[8]
+  f.time = time;
+  f.dxe = c->d[0];
+  f.dye = c->d[1];
+  f.dze = c->d[2];
+  f.xMax = p.xMax;
+  f.yMax = p.dimnum > 1 ? p.yMax : 0.0;
+  f.zMax = p.dimnum > 2 ? p.zMax : 0.0;
+  for (int a = 0; a < 3; ++a) {
+    (&f.xg)[a] = c->dgrid[a];
+    (&f.pxg)[a] = c->dgrid[a + 3];
+  }
+  f.Vt = c->Vt;
+  f.Kt = c->Kt;
+  for (int a = 0; a < 3; ++a) f.At[a] = c->At[a];
+  f.veq = c->veq;
+  return f;
+}
+
+// Sweeps of step i (src/evolution.cu:727-870).  NORM is set on the last sweep
+// of an imaginary-time step; SCALE_IN on the first when a norm is pending.
+static void schedule(const gpue_ctx* c, int gstate, int i, bool pending, std::vector<Sweep>& out) {
+  out.clear();
+  const gpue_params& p = c->p;
+  const int nx = c->dims[0], ny = c->dims[1], nz = c->dims[2];
+  const double invN = 1.0 / (double)c->N;
+  const unsigned scale = pending ? F_SCALE_IN : 0u;
+  const bool flip = (i % 2) != 0;
+  if (p.dimnum == 1) {
+    unsigned f = scale | F_HALFV_PRE | F_HALFV_POST | (p.corotating ? F_GAUGE_POST : 0u);
+    out.push_back({true, 0, RM_FULL, f, invN});
+  } else {
+    out.push_back({true, 0, RM_FWD, scale | F_HALFV_PRE, 0.0});
+    if (p.dimnum == 2) {
+      out.push_back({false, 1, CM_K, 0u, invN});
+    } else {
+      out.push_back({false, 1, CM_FWD, 0u, 0.0});
+      out.push_back({false, 2, CM_K, 0u, invN});
+      out.push_back({false, 1, CM_INV, 0u, 0.0});
+    }
+    const bool zg = p.dimnum == 3 && !c->az_zero;
+    if (!p.corotating) {
+      out.push_back({true, 0, RM_INV, F_HALFV_POST, 0.0});
+    } else if (flip) {   // x -> y (-> z)
+      out.push_back({true, 0, RM_INV, F_HALFV_POST | F_GAUGE_POST, 1.0 / nx});
+      out.push_back({false, 1, CM_GAUGE, 0u, 1.0 / ny});
+      if (zg) out.push_back({false, 2, CM_GAUGE, 0u, 1.0 / nz});
+    } else {             // (z ->) y -> x
+      out.push_back({true, 0, RM_INV, F_HALFV_POST, 0.0});
+      if (zg) out.push_back({false, 2, CM_GAUGE, 0u, 1.0 / nz});
+      out.push_back({false, 1, CM_GAUGE, 0u, 1.0 / ny});
+      out.push_back({true, 0, RM_GAUGE, 0u, 1.0 / nx});
+    }
+  }
+  if (gstate == 0) out.back().flags |= F_NORM;
+}
+
+static std::string sweep_name(const Sweep& s) {
+  static const char* rm[] = {"row_fwd", "row_inv", "row_gauge", "row_full", "row_ksum"};
+  static const char* cm[] = {"fwd", "inv", "k", "gauge", "ksum"};
+  std::string n = s.row ? rm[s.mode] : (std::string(s.axis == 1 ? "coly_" : "colz_") + cm[s.mode]);
+  if (s.flags & F_HALFV_PRE) n += "+vpre";
+  if (s.flags & F_HALFV_POST) n += "+vpost";
+  if (s.flags & F_GAUGE_POST) n += "+gx";
+  if (s.flags & F_NORM) n += "+norm";
+  return n;
+}
+
+static int kind_index(gpue_ctx* c, const std::string& n) {
+  for (size_t k = 0; k < c->kind_names.size(); ++k)
+    if (c->kind_names[k] == n) return (int)k;
+  c->kind_names.push_back(n);
+  c->kind_n.push_back(0);
+  c->kind_ms.push_back(0.0);
+  return (int)c->kind_names.size() - 1;
+}
+
+static gpue_status timing_flush(gpue_ctx* c) {
+  if (c->ev_used == 0) return GPUE_OK;
+  HIPCHK(c, hipEventSynchronize(c->ev[2 * (c->ev_used - 1) + 1]));
+  for (int k = 0; k < c->ev_used; ++k) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]));
+    c->kind_ms[c->ev_kind[k]] += ms;
+    c->kind_n[c->ev_kind[k]] += 1;
+  }
+  c->ev_used = 0;
+  return GPUE_OK;
+}
+
+static gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void* in, void* out) {
+  const int n = c->dims[s.axis];
+  SweepArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in;
+  a.out = out;
+  a.tw = c->tw[n];
+  a.ph = ph;
+  a.lines = c->dims[1] * c->dims[2];
+  a.axis = s.axis;
+  a.gauge_axis = s.axis;
+  a.flags = s.flags;
+  a.invN = s.invN;
+  a.norm_in = c->results + 0;
+  a.partials = c->partials;
+  a.counter = c->counter;
+  a.result = (s.mode == (s.row ? (int)RM_KSUM : (int)CM_KSUM)) ? c->results + 1 : c->results + 0;
+  a.dV = c->d[0] * c->d[1] * c->d[2];
+  int slot = -1;
+  if (c->timing) {
+    if (c->ev_used == TIMING_PAIRS) {
+      gpue_status st = timing_flush(c);
+      if (st != GPUE_OK) return st;
+    }
+    slot = c->ev_used++;
+    c->ev_kind[slot] = kind_index(c, sweep_name(s));
+    HIPCHK(c, hipEventRecord(c->ev[2 * slot], c->stream));
+  }
+  bool handled = false;
+  hipError_t e = launch_sweep(c->prec, s.row, n, s.mode, a, c->stream, &handled);
+  if (!handled) return fail(c, GPUE_EUNSUPPORTED, "no sweep kernel for n=" + std::to_string(n));
+  if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("sweep launch: ") + hipGetErrorString(e));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2 * slot + 1], c->stream));
+  return GPUE_OK;
+}
+
+static gpue_status apply_pending(gpue_ctx* c) {
+  if (!c->pending_norm) return GPUE_OK;
+  hipError_t e = c->prec == 64 ? launch_apply_norm<double>(c->psi, c->N, c->results, c->stream)
+                               : launch_apply_norm<float>(c->psi, c->N, c->results, c->stream);
+  if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
+  c->pending_norm = false;
+  return GPUE_OK;
+}
+
+static bool valid_dim(int n) { return sweep_supported(n); }
+
+// ---------------------------------------------------------------------------
+// ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+gpue_status gpue_params_default(gpue_params* p) {
+  if (!p) return GPUE_EINVAL;
+  std::memset(p, 0, sizeof(*p));
+  // src/parser.cu:14-67
+  p->dimnum = 2;
+  p->xDim = p->yDim = p->zDim = 256;
+  p->omega = 0.0;
+  p->gammaY = 1.0;
+  p->gdt = 1e-4;
+  p->dt = 1e-4;
+  p->omegaX = p->omegaY = p->omegaZ = 6.283;
+  p->gpe = 0;
+  p->corotating = 0;
+  p->a_kind = GPUE_A_ROTATION;
+  p->precision = 64;
+  // src/init.cu:98-137 with atoms = 1, interaction = 1, box_size < 0
+  const double HBAR = 1.05457148e-34, PI = 3.141592653589793;
+  const double mass = 1.4431607e-25, a_s = 4.76e-9;
+  const int atoms = 1;
+  p->hbar = HBAR;
+  p->mass = mass;
+  double gDenConst = atoms * 4 * HBAR * HBAR * PI * (a_s / mass);
+  gDenConst *= std::sqrt(mass * (p->omegaZ) / (2 * PI * HBAR));   // 2D
+  p->g = 1.0 * gDenConst;
+  double Rxy = std::pow(15, 0.2) * std::pow(atoms * a_s * std::sqrt(mass * p->omegaZ / HBAR), 0.2);
+  double a0x = std::sqrt(HBAR / (2 * mass * p->omegaX));
+  double a0y = std::sqrt(HBAR / (2 * mass * p->omegaY));
+  double a0z = std::sqrt(HBAR / (2 * mass * p->omegaZ));
+  p->xMax = 6 * Rxy * a0x;
+  p->yMax = 6 * Rxy * a0y;
+  p->zMax = 6 * Rxy * a0z;
+  return GPUE_OK;
+}
+
+const char* gpue_last_error(const gpue_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void gpue_destroy(gpue_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* bufs[] = {c->psi, c->tmp, c->partials, c->counter, c->results, c->Vt, c->Kt, c->At[0], c->At[1],
+                  c->At[2], c->veq, c->vmask, c->vcand, c->vmarker, c->vfound};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (int a = 0; a < 6; ++a)
+    if (c->dgrid[a]) (void)hipFree(c->dgrid[a]);
+  for (auto& kv : c->tw) (void)hipFree(kv.second);
+  for (auto& e : c->ev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
+  if (!pin || !out) return GPUE_EINVAL;
+  *out = nullptr;
+  gpue_ctx* c = new gpue_ctx();
+  c->p = *pin;
+  gpue_params& p = c->p;
+  auto bail = [&](gpue_status s, const std::string& m) {
+    c->err = m;
+    static thread_local std::string last;
+    last = m;
+    std::fprintf(stderr, "gpue_create: %s\n", m.c_str());
+    gpue_destroy(c);
+    return s;
+  };
+  if (p.dimnum < 1 || p.dimnum > 3) return bail(GPUE_EINVAL, "dimnum must be 1, 2 or 3");
+  if (p.precision != 64 && p.precision != 32) return bail(GPUE_EINVAL, "precision must be 64 or 32");
+  if (p.dimnum < 3) p.zDim = 1;
+  if (p.dimnum < 2) p.yDim = 1;
+  c->dims[0] = p.xDim;
+  c->dims[1] = p.yDim;
+  c->dims[2] = p.zDim;
+  for (int a = 0; a < p.dimnum; ++a)
+    if (!valid_dim(c->dims[a]))
+      return bail(GPUE_EUNSUPPORTED, "unsupported grid size " + std::to_string(c->dims[a]) +
+                                         " (supported: 16 32 64 128 192 256 384 512 1024)");
+  if (!(p.hbar > 0) || !(p.mass > 0) || !(p.xMax > 0) || (p.dimnum > 1 && !(p.yMax > 0)) ||
+      (p.dimnum > 2 && !(p.zMax > 0)))
+    return bail(GPUE_EINVAL, "hbar, mass and box sizes must be positive");
+  if (p.a_kind != GPUE_A_ROTATION && p.a_kind != GPUE_A_CONSTANT) return bail(GPUE_EINVAL, "bad a_kind");
+  c->prec = p.precision;
+  c->esz = p.precision == 64 ? 16 : 8;
+  c->N = (long)p.xDim * p.yDim * p.zDim;
+  c->a_kind = p.a_kind == GPUE_A_ROTATION ? A_ROTATION : A_ZERO;
+  make_grids(p.dimnum, p.xDim, p.yDim, p.zDim, p.xMax, p.yMax, p.zMax, c->grid, c->d);
+
+  hipError_t e = hipSetDevice(p.device);
+  if (e != hipSuccess) return bail(GPUE_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(GPUE_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  if (hipMalloc(&c->psi, c->N * c->esz) != hipSuccess || hipMalloc(&c->tmp, c->N * c->esz) != hipSuccess)
+    return bail(GPUE_ENOMEM, "psi buffers");
+  for (int a = 0; a < 6; ++a) {
+    c->dgrid[a] = dev_copy(c->grid[a], &e);
+    if (e != hipSuccess) return bail(GPUE_ENOMEM, "grid arrays");
+  }
+  for (int a = 0; a < p.dimnum; ++a) {
+    int n = c->dims[a];
+    if (c->tw.count(n)) continue;
+    std::vector<double> re, im;
+    twiddles(n, re, im);
+    void* t = nullptr;
+    if (hipMalloc(&t, (size_t)n * c->esz) != hipSuccess) return bail(GPUE_ENOMEM, "twiddles");
+    c->tw[n] = t;
+    if (c->prec == 64) {
+      std::vector<double> il(2 * n);
+      for (int m = 0; m < n; ++m) { il[2 * m] = re[m]; il[2 * m + 1] = im[m]; }
+      e = hipMemcpy(t, il.data(), il.size() * sizeof(double), hipMemcpyHostToDevice);
+    } else {
+      std::vector<float> il(2 * n);
+      for (int m = 0; m < n; ++m) { il[2 * m] = (float)re[m]; il[2 * m + 1] = (float)im[m]; }
+      e = hipMemcpy(t, il.data(), il.size() * sizeof(float), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) return bail(GPUE_EHIP, "twiddle upload");
+  }
+  // reduction scratch: the largest grid of any reducing kernel, times 4 accumulators
+  size_t nb = REDUCE_BLOCKS;
+  nb = std::max<size_t>(nb, sweep_blocks(true, p.xDim, p.yDim * p.zDim, 1));
+  if (p.dimnum > 1) nb = std::max<size_t>(nb, sweep_blocks(false, p.yDim, p.xDim, p.zDim));
+  if (p.dimnum > 2) nb = std::max<size_t>(nb, sweep_blocks(false, p.zDim, p.xDim, p.yDim));
+  c->partials_n = nb * 4;
+  if (hipMalloc((void**)&c->partials, c->partials_n * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->counter, 64) != hipSuccess || hipMalloc((void**)&c->results, 16 * sizeof(double)) != hipSuccess)
+    return bail(GPUE_ENOMEM, "reduction scratch");
+  if (hipMemset(c->counter, 0, 64) != hipSuccess || hipMemset(c->results, 0, 16 * sizeof(double)) != hipSuccess ||
+      hipMemset(c->psi, 0, c->N * c->esz) != hipSuccess)
+    return bail(GPUE_EHIP, "memset");
+  c->ev.resize(2 * TIMING_PAIRS);
+  c->ev_kind.resize(TIMING_PAIRS);
+  for (auto& ev : c->ev)
+    if (hipEventCreate(&ev) != hipSuccess) return bail(GPUE_EHIP, "event create");
+  if (hipDeviceSynchronize() != hipSuccess) return bail(GPUE_EHIP, "device sync");
+  *out = c;
+  return GPUE_OK;
+}
+
+gpue_status gpue_get_grid(gpue_ctx* c, int axis, double* out) {
+  if (!c || !out || axis < 0 || axis > 5) return GPUE_EINVAL;
+  std::memcpy(out, c->grid[axis].data(), c->grid[axis].size() * sizeof(double));
+  return GPUE_OK;
+}
+
+gpue_status gpue_set_g(gpue_ctx* c, double g) {
+  if (!c) return GPUE_EINVAL;
+  c->p.g = g;
+  return GPUE_OK;
+}
+
+gpue_status gpue_sync(gpue_ctx* c) {
+  if (!c) return GPUE_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GPUE_OK;
+}
+
+static gpue_status set_table(gpue_ctx* c, int which, const double* src, bool from_device) {
+  double** slot = which == GPUE_FIELD_V ? &c->Vt : which == GPUE_FIELD_K ? &c->Kt : &c->At[which - GPUE_FIELD_AX];
+  if (!*slot) {
+    if (hipMalloc((void**)slot, c->N * sizeof(double)) != hipSuccess) return fail(c, GPUE_ENOMEM, "field table");
+  }
+  HIPCHK(c, hipMemcpyAsync(*slot, src, c->N * sizeof(double), from_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (which == GPUE_FIELD_V) c->v_kind = V_TABLE;
+  else if (which == GPUE_FIELD_K) c->k_kind = K_TABLE;
+  else {
+    // the other components keep their current values: materialise them as tables
+    if (c->a_kind != A_TABLE) {
+      for (int a = 0; a < 3; ++a) {
+        if (a == which - GPUE_FIELD_AX) continue;
+        if (!c->At[a] && hipMalloc((void**)&c->At[a], c->N * sizeof(double)) != hipSuccess)
+          return fail(c, GPUE_ENOMEM, "field table");
+        std::vector<double> h(c->N, 0.0);
+        if (c->a_kind == A_ROTATION) {
+          for (long gid = 0; gid < c->N; ++gid) {
+            long xi = gid % c->dims[0], yi = (gid / c->dims[0]) % c->dims[1];
+            if (a == 0) h[gid] = -c->grid[1][yi] * c->p.omega * c->p.omegaX;
+            if (a == 1) h[gid] = c->grid[0][xi] * c->p.omega * c->p.omegaY;
+          }
+        }
+        HIPCHK(c, hipMemcpy(c->At[a], h.data(), c->N * sizeof(double), hipMemcpyHostToDevice));
+      }
+      c->a_kind = A_TABLE;
+    }
+    if (which == GPUE_FIELD_AZ) {
+      std::vector<double> h(c->N);
+      HIPCHK(c, hipMemcpy(h.data(), c->At[2], c->N * sizeof(double), hipMemcpyDeviceToHost));
+      c->az_zero = std::all_of(h.begin(), h.end(), [](double v) { return v == 0.0; });
+    }
+  }
+  return GPUE_OK;
+}
+
+gpue_status gpue_set_field(gpue_ctx* c, int which, const double* host_N) {
+  if (!c || !host_N || which < GPUE_FIELD_V || which > GPUE_FIELD_AZ) return GPUE_EINVAL;
+  return set_table(c, which, host_N, false);
+}
+
+static std::map<std::string, double> name_map(const char* const* names, const double* values, int n) {
+  std::map<std::string, double> m;
+  for (int i = 0; i < n; ++i)
+    if (names && names[i]) m[names[i]] = values[i];
+  return m;
+}
+
+gpue_status gpue_set_field_eqn(gpue_ctx* c, int which, const char* expr, const char* const* names,
+                               const double* values, int n_names) {
+  if (!c || !expr || which < GPUE_FIELD_V || which > GPUE_FIELD_AZ) return GPUE_EINVAL;
+  std::vector<EqnNodeC> nodes;
+  bool td = false;
+  std::string err;
+  if (eqn_parse(expr, name_map(names, values, n_names), {}, nodes, td, err)) return fail(c, GPUE_EPARSE, err);
+  EqnProgH prog;
+  if (eqn_to_program(nodes.data(), (int)nodes.size(), prog, err)) return fail(c, GPUE_EPARSE, err);
+  static_assert(sizeof(EqnProgH) == sizeof(EqnProg), "program layouts differ");
+  EqnProg dp;
+  std::memcpy(&dp, &prog, sizeof(dp));
+  const gpue_params& p = c->p;
+  const double xM = p.xMax, yM = p.dimnum > 1 ? p.yMax : 0.0, zM = p.dimnum > 2 ? p.zMax : 0.0;
+  if (td) {
+    if (which != GPUE_FIELD_V)
+      return fail(c, GPUE_EUNSUPPORTED, "time-dependent K and A fields are not supported (V only)");
+    if (!c->veq && hipMalloc((void**)&c->veq, sizeof(EqnProg)) != hipSuccess) return fail(c, GPUE_ENOMEM, "eqn");
+    HIPCHK(c, hipMemcpy(c->veq, &dp, sizeof(EqnProg), hipMemcpyHostToDevice));
+    c->v_kind = V_EQN;
+    return GPUE_OK;
+  }
+  // static: evaluate once on the device (find_field), then use as a table
+  double* buf = nullptr;
+  if (hipMalloc((void**)&buf, c->N * sizeof(double)) != hipSuccess) return fail(c, GPUE_ENOMEM, "eqn field");
+  hipError_t e = launch_eqn_field(dp, buf, c->dims[0], c->dims[1], c->dims[2], c->d[0], c->d[1], c->d[2], xM, yM, zM,
+                                  0.0, c->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(buf);
+    return fail(c, GPUE_EHIP, hipGetErrorString(e));
+  }
+  gpue_status st = set_table(c, which, buf, true);
+  (void)hipFree(buf);
+  return st;
+}
+
+gpue_status gpue_upload_wfc(gpue_ctx* c, const double* il) {
+  if (!c || !il) return GPUE_EINVAL;
+  if (c->prec == 64) {
+    HIPCHK(c, hipMemcpyAsync(c->psi, il, c->N * 16, hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::vector<float> f(2 * c->N);
+    for (long i = 0; i < 2 * c->N; ++i) f[i] = (float)il[i];
+    HIPCHK(c, hipMemcpyAsync(c->psi, f.data(), c->N * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->pending_norm = false;
+  return GPUE_OK;
+}
+
+gpue_status gpue_download_wfc(gpue_ctx* c, double* il) {
+  if (!c || !il) return GPUE_EINVAL;
+  gpue_status st = apply_pending(c);
+  if (st != GPUE_OK) return st;
+  if (c->prec == 64) {
+    HIPCHK(c, hipMemcpyAsync(il, c->psi, c->N * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> f(2 * c->N);
+    HIPCHK(c, hipMemcpyAsync(f.data(), c->psi, c->N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (long i = 0; i < 2 * c->N; ++i) il[i] = f[i];
+  }
+  return GPUE_OK;
+}
+
+gpue_status gpue_parsum(gpue_ctx* c) {
+  if (!c) return GPUE_EINVAL;
+  gpue_status st = apply_pending(c);
+  if (st != GPUE_OK) return st;
+  const double dV = c->d[0] * c->d[1] * c->d[2];
+  hipError_t e = c->prec == 64
+                     ? launch_norm_sum<double>(c->psi, c->N, dV, c->partials, c->counter, c->results, c->stream)
+                     : launch_norm_sum<float>(c->psi, c->N, dV, c->partials, c->counter, c->results, c->stream);
+  if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
+  c->pending_norm = true;
+  return apply_pending(c);
+}
+
+gpue_status gpue_energy(gpue_ctx* c, double* energy, double* mu) {
+  if (!c) return GPUE_EINVAL;
+  gpue_status st = apply_pending(c);
+  if (st != GPUE_OK) return st;
+  const gpue_params& p = c->p;
+  Phys ph = make_phys(c, 0, 0.0);
+  const double invN = 1.0 / (double)c->N;
+  // kinetic term by Parseval over the full transform
+  if (p.dimnum == 1) {
+    st = launch(c, {true, 0, RM_KSUM, 0u, invN}, ph, c->psi, c->psi);
+  } else {
+    st = launch(c, {true, 0, RM_FWD, 0u, 0.0}, ph, c->psi, c->tmp);
+    if (st == GPUE_OK && p.dimnum == 3) st = launch(c, {false, 1, CM_FWD, 0u, 0.0}, ph, c->tmp, c->tmp);
+    if (st == GPUE_OK) st = launch(c, {false, p.dimnum == 2 ? 1 : 2, CM_KSUM, 0u, invN}, ph, c->tmp, c->tmp);
+  }
+  if (st != GPUE_OK) return st;
+  const int gauge = p.corotating && p.dimnum > 1;
+  hipError_t e = c->prec == 64 ? launch_energy_pos<double>(c->psi, ph, gauge, p.hbar, c->d[0], c->d[1], c->d[2],
+                                                           c->partials, c->counter, c->results + 2, c->stream)
+                               : launch_energy_pos<float>(c->psi, ph, gauge, p.hbar, c->d[0], c->d[1], c->d[2],
+                                                          c->partials, c->counter, c->results + 2, c->stream);
+  if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
+  double r[6];
+  HIPCHK(c, hipMemcpyAsync(r, c->results, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double dV = c->d[0] * c->d[1] * c->d[2];
+  const double kin = r[1], vt = r[2], in4 = r[3], lz = r[4];
+  double E = p.gpe ? (kin + vt + 0.5 * p.g * in4 + lz) * dV : (kin + vt + lz) * dV;
+  double M = p.gpe ? (kin + vt + p.g * in4 + lz) * dV : (kin + vt + lz) * dV;
+  if (energy) *energy = E;
+  if (mu) *mu = M;
+  if (!std::isfinite(E)) return fail(c, GPUE_ENONFINITE, "energy is not finite");
+  return GPUE_OK;
+}
+
+gpue_status gpue_evolve(gpue_ctx* c, int first_step, int nsteps, int gstate, int energy_every, double energy_thr,
+                        double* energies, int max_energies, gpue_evolve_result* res) {
+  if (!c || nsteps < 0 || (gstate != 0 && gstate != 1) || first_step < 0) return GPUE_EINVAL;
+  gpue_evolve_result r;
+  std::memset(&r, 0, sizeof(r));
+  std::vector<Sweep> sched;
+  const double Dt = gstate == 0 ? c->p.gdt : c->p.dt;
+  int s;
+  for (s = 0; s < nsteps; ++s) {
+    const int i = first_step + s;
+    Phys ph = make_phys(c, gstate, Dt * i);   // time = Dt*i (src/evolution.cu:408)
+    schedule(c, gstate, i, c->pending_norm, sched);
+    for (const Sweep& sw : sched) {
+      gpue_status st = launch(c, sw, ph, c->psi, c->psi);
+      if (st != GPUE_OK) return st;
+    }
+    if (gstate == 0) c->pending_norm = true;
+    if (energy_every > 0 && i % energy_every == 0) {
+      double E = 0, M = 0;
+      gpue_status st = gpue_energy(c, &E, &M);
+      if (st != GPUE_OK) return st;
+      if (energies && r.n_energies < max_energies) energies[r.n_energies] = E;
+      r.n_energies++;
+      r.energy = E;
+      r.mu = M;
+      const double old = (i != 0) ? c->last_energy : 0.0;
+      c->last_energy = E;
+      if (i != 0 && std::fabs(old - E) < energy_thr * old && gstate == 0) {   // src/evolution.cu:899
+        r.early_stop = 1;
+        ++s;
+        break;
+      }
+    }
+  }
+  r.steps_done = s;
+  if (c->timing) {
+    gpue_status st = timing_flush(c);
+    if (st != GPUE_OK) return st;
+  }
+  if (gstate == 0 && c->pending_norm) {
+    double nrm = 0;
+    HIPCHK(c, hipMemcpyAsync(&nrm, c->results, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!std::isfinite(nrm) || nrm == 0.0) return fail(c, GPUE_ENONFINITE, "norm is not finite");
+  }
+  if (res) *res = r;
+  return GPUE_OK;
+}
+
+gpue_status gpue_vortex_marker_2d(gpue_ctx* c, double radius, int* marker_out, int* found) {
+  if (!c || !marker_out) return GPUE_EINVAL;
+  if (c->p.dimnum != 2 || c->dims[0] != c->dims[1])
+    return fail(c, GPUE_EUNSUPPORTED, "findVortex needs a square 2D grid (src/tracker.cu:111-113)");
+  gpue_status st = apply_pending(c);
+  if (st != GPUE_OK) return st;
+  const int xDim = c->dims[0];
+  const size_t np = (size_t)(xDim - 1) * (xDim - 1);
+  if (!c->vmask) {
+    if (hipMalloc((void**)&c->vmask, np) != hipSuccess || hipMalloc((void**)&c->vcand, np * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&c->vmarker, (size_t)xDim * xDim * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&c->vfound, 64) != hipSuccess)
+      return fail(c, GPUE_ENOMEM, "vortex scratch");
+  }
+  if (radius != c->vmask_radius) {
+    std::vector<unsigned char> m;
+    vortex_mask(c->grid[0].data(), xDim, radius, m);
+    HIPCHK(c, hipMemcpy(c->vmask, m.data(), np, hipMemcpyHostToDevice));
+    c->vmask_radius = radius;
+  }
+  hipError_t e = c->prec == 64 ? launch_vortex<double>(c->psi, c->vmask, xDim, c->vcand, c->vmarker, c->vfound, c->stream)
+                               : launch_vortex<float>(c->psi, c->vmask, xDim, c->vcand, c->vmarker, c->vfound, c->stream);
+  if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
+  int f = 0;
+  HIPCHK(c, hipMemcpyAsync(marker_out, c->vmarker, (size_t)xDim * xDim * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&f, c->vfound, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (found) *found = f;
+  return GPUE_OK;
+}
+
+gpue_status gpue_vortex_list_2d(gpue_ctx* c, double radius, int* ijw, int max_v, int* n_v) {
+  if (!c) return GPUE_EINVAL;
+  if (c->p.dimnum != 2) return fail(c, GPUE_EUNSUPPORTED, "2D only");
+  const int xDim = c->dims[0];
+  std::vector<int> m((size_t)xDim * xDim);
+  int found = 0;
+  gpue_status st = gpue_vortex_marker_2d(c, radius, m.data(), &found);
+  if (st != GPUE_OK) return st;
+  int n = 0;   // vortPos (src/tracker.cu:212-229): raster order, |marker| >= 1
+  for (int i = 0; i < xDim; ++i)
+    for (int j = 0; j < xDim; ++j) {
+      int v = m[(size_t)i * xDim + j];
+      if (std::abs(v) >= 1) {
+        if (ijw && n < max_v) {
+          ijw[3 * n] = i;
+          ijw[3 * n + 1] = j;
+          ijw[3 * n + 2] = v;
+        }
+        ++n;
+      }
+    }
+  if (n_v) *n_v = n;
+  return GPUE_OK;
+}
+
+gpue_status gpue_set_timing(gpue_ctx* c, int on) {
+  if (!c) return GPUE_EINVAL;
+  c->timing = on != 0;
+  c->ev_used = 0;
+  c->kind_names.clear();
+  c->kind_n.clear();
+  c->kind_ms.clear();
+  return GPUE_OK;
+}
+
+gpue_status gpue_timing_get(gpue_ctx* c, int kind, const char** name, long* launches, double* ms) {
+  if (!c || kind < 0) return GPUE_EINVAL;
+  if (kind >= (int)c->kind_names.size()) return GPUE_EINVAL;
+  if (name) *name = c->kind_names[kind].c_str();
+  if (launches) *launches = c->kind_n[kind];
+  if (ms) *ms = c->kind_ms[kind];
+  return GPUE_OK;
+}
+
+gpue_status gpue_step_bytes(gpue_ctx* c, int gstate, double* bytes, double* sweeps) {
+  if (!c) return GPUE_EINVAL;
+  std::vector<Sweep> a, b;
+  schedule(c, gstate, 0, false, a);
+  schedule(c, gstate, 1, false, b);
+  const double P = 0.5 * (double)(a.size() + b.size());
+  if (sweeps) *sweeps = P;
+  if (bytes) *bytes = 2.0 * (double)c->esz * (double)c->N * P;
+  return GPUE_OK;
+}
+
+gpue_status gpue_eqn_parse(const char* expr, const char* const* names, const double* values, int n_names,
+                           gpue_eqn_node* nodes, int max_nodes, int* n_nodes, int* time_dependent) {
+  if (!expr) return GPUE_EINVAL;
+  std::vector<EqnNodeC> out;
+  bool td = false;
+  std::string err;
+  if (eqn_parse(expr, name_map(names, values, n_names), {}, out, td, err)) {
+    std::fprintf(stderr, "gpue_eqn_parse: %s\n", err.c_str());
+    return GPUE_EPARSE;
+  }
+  static_assert(sizeof(gpue_eqn_node) == sizeof(EqnNodeC), "node layouts differ");
+  if (n_nodes) *n_nodes = (int)out.size();
+  if (time_dependent) *time_dependent = td ? 1 : 0;
+  if (nodes) std::memcpy(nodes, out.data(), std::min<size_t>(max_nodes, out.size()) * sizeof(EqnNodeC));
+  return (int)out.size() > max_nodes && nodes ? GPUE_EINVAL : GPUE_OK;
+}
+
+double gpue_eqn_eval(const gpue_eqn_node* nodes, int n, double x, double y, double z, double t) {
+  return eqn_eval(reinterpret_cast<const EqnNodeC*>(nodes), n, x, y, z, t);
+}
+
+}  // extern "C"

@@ -1,0 +1,247 @@
+// misc_kernels.hip -- pointwise kernels around the sweeps: renormalisation,
+// the position-space part of GPUE's energy, the vortex plaquette detector and
+// the per-point equation evaluator.
+#include "misc_kernels.h"
+
+#include <math.h>
+
+namespace gpue {
+
+// find_field (src/dynamic.cu:605-616): static field at dx*xid - xMax, ...
+__global__ void k_eqn_field(EqnProg prog, double* out, int xDim, int yDim, int zDim, double dx, double dy,
+                            double dz, double xMax, double yMax, double zMax, double t) {
+  const long N = (long)xDim * yDim * zDim;
+  for (long gid = blockIdx.x * (long)blockDim.x + threadIdx.x; gid < N; gid += (long)gridDim.x * blockDim.x) {
+    const int xi = (int)(gid % xDim);
+    const int yi = (int)((gid / xDim) % yDim);
+    const int zi = (int)(gid / ((long)xDim * yDim));
+    out[gid] = eval_eqn(&prog, dx * xi - xMax, dy * yi - yMax, dz * zi - zMax, t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// psi /= norm  (pending renormalisation made explicit)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_apply_norm(cx<T>* psi, long N, const double* norm) {
+  const double nrm = *norm;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    cx<T> v = psi[i];
+    psi[i] = mk((T)((double)v.x / nrm), (T)((double)v.y / nrm));
+  }
+}
+
+// parSum on its own (src/split_op.cu:83-129): sum |psi|^2 then divide
+template <typename T>
+__global__ void k_norm_sum(const cx<T>* psi, long N, double dV, double* partials, unsigned* counter,
+                           double* norm_out) {
+  __shared__ double sh[64];
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    cx<T> v = psi[i];
+    acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+  }
+  double pv[1] = {acc}, tot[1];
+  if (grid_sum<1>(pv, partials, counter, sh, tot)) norm_out[0] = sqrt(tot[0] * dV);
+}
+
+// ---------------------------------------------------------------------------
+// Position-space terms of energy_calc (src/split_op.cu:275-360):
+//   [0] sum V |psi|^2      [1] sum |psi|^4
+//   [2] sum Re conj(psi) (i hbar) sum_a A_a D_a psi   (forward differences on the
+//       flat array, derive src/kernels.cu:58-71; included iff corotating && dimnum>1)
+//   [3] sum |psi|^2
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_energy_pos(const cx<T>* psi, Phys p, int gauge, double hbar, double dx, double dy,
+                             double dz, double* partials, unsigned* counter, double* out) {
+  __shared__ double sh[64];
+  const long N = (long)p.xDim * p.yDim * p.zDim;
+  const long sy = p.xDim, sz = (long)p.xDim * p.yDim;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long gid = blockIdx.x * (long)blockDim.x + threadIdx.x; gid < N; gid += (long)gridDim.x * blockDim.x) {
+    const int xi = (int)(gid % p.xDim);
+    const int yi = (int)((gid / p.xDim) % p.yDim);
+    const int zi = (int)(gid / sz);
+    const cx<T> w = psi[gid];
+    const double wx = w.x, wy = w.y;
+    const double dens = wx * wx + wy * wy;
+    acc[0] += V_val(p, xi, yi, zi, gid) * dens;
+    acc[1] += dens * dens;
+    acc[3] += dens;
+    if (gauge) {
+      double sx = 0.0, syy = 0.0;
+      const long strides[3] = {1, sy, sz};
+      const double h[3] = {dx, dy, dz};
+      const int naxes = p.dimnum;
+      for (int ax = 0; ax < naxes; ++ax) {
+        double ddx, ddy;
+        if (gid + strides[ax] < N) {
+          const cx<T> q = psi[gid + strides[ax]];
+          ddx = ((double)q.x - wx) / h[ax];
+          ddy = ((double)q.y - wy) / h[ax];
+        } else {
+          ddx = wx / h[ax];
+          ddy = wy / h[ax];
+        }
+        const double A = A_val(p, ax, xi, yi, zi, gid);
+        sx += ddx * A;
+        syy += ddy * A;
+      }
+      // Re[conj(psi) * (i hbar) * S] = hbar * (psi.y S.x - psi.x S.y)
+      acc[2] += wx * (-syy * hbar) - (-wy) * (sx * hbar);
+    }
+  }
+  double tot[4];
+  if (grid_sum<4>(acc, partials, counter, sh, tot)) {
+    out[0] = tot[0]; out[1] = tot[1]; out[2] = tot[2]; out[3] = tot[3];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Vortex plaquettes (src/tracker.cu:102-161 with src/minions.cu:62-93).
+// Pass 1: every in-mask plaquette computes its winding candidate (0 = none).
+// Pass 2: a candidate is accepted iff the number of consecutive candidates
+// immediately before it in the raster stream of in-mask plaquettes is even --
+// the closed form of GPUE's sequential cond_x/cond_y skip rule.
+// Arithmetic is kept unfused so that it rounds as GPUE's host code does.
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+__device__ __forceinline__ double2 h_scale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double h_mag(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
+__device__ __forceinline__ double2 h_div(double2 num, double2 den) {
+  double2 c = make_double2(den.x, -den.y);
+  double2 m = make_double2(num.x * c.x - num.y * c.y, num.x * c.y + num.y * c.x);
+  return h_scale(m, 1.0 / (den.x * den.x + den.y * den.y));
+}
+__device__ __forceinline__ double h_phase(double2 a, double2 b) {
+  const double PI_ = 3.141592653589793;
+  double2 g = h_scale(h_div(a, b), h_mag(b) / h_mag(a));
+  double ph = atan2(g.y, g.x);
+  if (ph <= -PI_) ph += 2 * PI_;
+  return ph;
+}
+
+template <typename T>
+__global__ void k_vortex_cand(const cx<T>* psi, const unsigned char* mask, int xDim, int* cand) {
+  const int P = xDim - 1;
+  const long np = (long)P * P;
+  const double PI_ = 3.141592653589793;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < np; idx += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / P), j = (int)(idx % P);
+    int c = 0;
+    if (mask[idx]) {
+      auto ld = [&](int a, int b) {
+        cx<T> v = psi[(long)a * xDim + b];
+        return make_double2((double)v.x, (double)v.y);
+      };
+      double2 a = ld(i, j), b = ld(i + 1, j), cc = ld(i + 1, j + 1), d = ld(i, j + 1);
+      double sum = h_phase(a, b) + h_phase(b, cc);
+      sum = sum + h_phase(cc, d);
+      sum = sum + h_phase(d, a);
+      long rnd = lround(sum / (2 * PI_));
+      // |sum| >= 1.9 pi rounds to a winding of magnitude >= 1, so hits are nonzero
+      if (sum >= 1.9 * PI_) c = (int)rnd;
+      else if (sum <= -1.9 * PI_) c = (int)-rnd;
+    } else {
+      c = INT_MAX;   // not in mask: not part of the stream
+    }
+    cand[idx] = c;
+  }
+}
+#pragma clang fp contract(on)
+
+__global__ void k_vortex_resolve(const int* cand, int xDim, int* marker, int* found) {
+  const int P = xDim - 1;
+  const long np = (long)P * P;
+  int local = 0;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < np; idx += (long)gridDim.x * blockDim.x) {
+    const int c = cand[idx];
+    if (c == 0 || c == INT_MAX) continue;
+    // count consecutive candidates before idx in the in-mask stream
+    int run = 0;
+    for (long q = idx - 1; q >= 0; --q) {
+      const int cq = cand[q];
+      if (cq == INT_MAX) continue;     // outside the mask
+      if (cq == 0) break;              // in-mask, no candidate: run ends
+      ++run;
+    }
+    if ((run & 1) == 0) {
+      const int i = (int)(idx / P), j = (int)(idx % P);
+      marker[(long)i * xDim + j] = c;
+      ++local;
+    }
+  }
+  if (local) atomicAdd(found, local);
+}
+
+__global__ void k_fill_int(int* p, long n, int v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+static inline int gs_blocks(long n, int bs, int cap) {
+  long b = (n + bs - 1) / bs;
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+hipError_t launch_eqn_field(const EqnProg& prog, double* out, int xDim, int yDim, int zDim, double dx, double dy,
+                            double dz, double xMax, double yMax, double zMax, double t, hipStream_t s) {
+  long N = (long)xDim * yDim * zDim;
+  hipLaunchKernelGGL(k_eqn_field, dim3(gs_blocks(N, 256, 8192)), dim3(256), 0, s, prog, out, xDim, yDim, zDim, dx, dy,
+                     dz, xMax, yMax, zMax, t);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_apply_norm(void* psi, long N, const double* norm, hipStream_t s) {
+  hipLaunchKernelGGL(k_apply_norm<T>, dim3(gs_blocks(N, 256, 16384)), dim3(256), 0, s, (cx<T>*)psi, N, norm);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_norm_sum(const void* psi, long N, double dV, double* partials, unsigned* counter, double* norm_out,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_norm_sum<T>, dim3(gs_blocks(N, 256, REDUCE_BLOCKS)), dim3(256), 0, s, (const cx<T>*)psi, N, dV,
+                     partials, counter, norm_out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_energy_pos(const void* psi, const Phys& p, int gauge, double hbar, double dx, double dy, double dz,
+                             double* partials, unsigned* counter, double* out, hipStream_t s) {
+  long N = (long)p.xDim * p.yDim * p.zDim;
+  hipLaunchKernelGGL(k_energy_pos<T>, dim3(gs_blocks(N, 256, REDUCE_BLOCKS)), dim3(256), 0, s, (const cx<T>*)psi, p,
+                     gauge, hbar, dx, dy, dz, partials, counter, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_vortex(const void* psi, const unsigned char* mask, int xDim, int* cand, int* marker, int* found,
+                         hipStream_t s) {
+  long np = (long)(xDim - 1) * (xDim - 1);
+  long nm = (long)xDim * xDim;
+  hipLaunchKernelGGL(k_fill_int, dim3(gs_blocks(nm, 256, 4096)), dim3(256), 0, s, marker, nm, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, found, 1L, 0);
+  hipLaunchKernelGGL(k_vortex_cand<T>, dim3(gs_blocks(np, 256, 8192)), dim3(256), 0, s, (const cx<T>*)psi, mask, xDim,
+                     cand);
+  hipLaunchKernelGGL(k_vortex_resolve, dim3(gs_blocks(np, 256, 8192)), dim3(256), 0, s, cand, xDim, marker, found);
+  return hipGetLastError();
+}
+
+template hipError_t launch_apply_norm<double>(void*, long, const double*, hipStream_t);
+template hipError_t launch_apply_norm<float>(void*, long, const double*, hipStream_t);
+template hipError_t launch_norm_sum<double>(const void*, long, double, double*, unsigned*, double*, hipStream_t);
+template hipError_t launch_norm_sum<float>(const void*, long, double, double*, unsigned*, double*, hipStream_t);
+template hipError_t launch_energy_pos<double>(const void*, const Phys&, int, double, double, double, double, double*,
+                                              unsigned*, double*, hipStream_t);
+template hipError_t launch_energy_pos<float>(const void*, const Phys&, int, double, double, double, double, double*,
+                                             unsigned*, double*, hipStream_t);
+template hipError_t launch_vortex<double>(const void*, const unsigned char*, int, int*, int*, int*, hipStream_t);
+template hipError_t launch_vortex<float>(const void*, const unsigned char*, int, int*, int*, int*, hipStream_t);
+
+}  // namespace gpue

@@ -1,0 +1,263 @@
+// sweeps.h -- fused split-operator sweeps for gfx950.
+//
+// One "sweep" reads psi once from HBM, applies a chain of 1-D transforms
+// along one axis with GPUE's pointwise operators fused between them, and
+// writes psi once.  GPUE's step (src/evolution.cu:727-870) maps onto
+//   2D: [V/2, FFTx] [FFTy, K, IFFTy] [IFFTx, V/2 (,x-gauge)] (+gauge sweeps)
+//   3D: [V/2, FFTx] [FFTy] [FFTz, K, IFFTz] [IFFTy] [IFFTx, V/2 (,x-gauge)] (+gauge)
+//   1D: [V/2, FFT, K, IFFT, V/2 (,gauge)]
+// The imaginary-time renormalisation (parSum, src/split_op.cu:83-129) is
+// fused: the last sweep of a step leaves per-workgroup partial sums of |psi|^2,
+// the last workgroup to finish reduces them in a fixed order to
+// norm = sqrt(sum * dV), and the next step's first sweep divides by it.
+// Operator fields are regenerated in-kernel from 1-D coordinate arrays
+// (harmonic V, rotation A, standard K) or read from tables (equation-string
+// or file fields), so the only per-step HBM traffic is psi itself.
+#pragma once
+#include "fft_core.h"
+
+namespace gpue {
+
+enum { V_HARMONIC = 0, V_TABLE = 1, V_EQN = 2 };
+enum { A_ROTATION = 0, A_ZERO = 1, A_TABLE = 2 };
+enum { K_STANDARD = 0, K_TABLE = 1 };
+
+// Sweep modes: row (x) and strided (y/z) kernels.
+enum { RM_FWD = 0, RM_INV = 1, RM_GAUGE = 2, RM_FULL = 3, RM_KSUM = 4 };
+enum { CM_FWD = 0, CM_INV = 1, CM_K = 2, CM_GAUGE = 3, CM_KSUM = 4 };
+
+// Flags of a sweep (runtime-uniform).
+enum : unsigned {
+  F_SCALE_IN = 1u,     // divide loaded psi by *norm_in (pending renormalisation)
+  F_HALFV_PRE = 2u,    // V/2 (+ nonlinearity) before the first transform
+  F_HALFV_POST = 4u,   // V/2 (+ nonlinearity) after the inverse transform
+  F_GAUGE_POST = 8u,   // extra [FFT, A_axis, IFFT] after the post op
+  F_NORM = 16u,        // accumulate |psi_out|^2 and produce norm_out
+};
+
+// Time-dependent equation program (postfix): CONST pushes val, X/Y/Z/T push a
+// coordinate, OP+k pops (a, b) and pushes GPUE operator k (src/dynamic.cu:568-600).
+enum : int { EQ_CONST = 0, EQ_X = 1, EQ_Y = 2, EQ_Z = 3, EQ_T = 4, EQ_OP = 16 };
+constexpr int EQ_MAX = 48;
+struct EqnProg {
+  int n;
+  int op[EQ_MAX];
+  double val[EQ_MAX];
+};
+
+struct Phys {
+  int xDim, yDim, zDim, dimnum;
+  int gstate, gpe;
+  int v_kind, a_kind, k_kind;
+  double cV;      // Dt / (2 hbar)          (GV/EV exponent, src/operators.cu:1081,1100)
+  double cH;      // (0.5*Dt) / hbar        (cMultDensity, src/kernels.cu:372)
+  double g;       // interaction * gDenConst
+  double cK;      // Dt / hbar              (GK/EK, src/operators.cu:1082,1102)
+  double Dt;      // gauge exponent scale   (GpA/EpA, src/operators.cu:1093-1110)
+  double hb2m;    // hbar^2 / (2 mass)      (simple_K, src/operators.cu:467)
+  double mass, omega, omegaX, omegaY, omegaZe, gammaY, x0, y0, z0e;
+  double time;    // Dt * i, for time-dependent fields
+  double dxe, dye, dze, xMax, yMax, zMax;   // coordinates for equation fields
+  const double *xg, *yg, *zg, *pxg, *pyg, *pzg;
+  const double *Vt, *Kt, *At[3];
+  const EqnProg* veq;  // device copy (V_EQN)
+};
+
+struct SweepArgs {
+  const void* in;
+  void* out;
+  const void* tw;
+  Phys ph;
+  int lines;          // row sweeps: yDim*zDim
+  int axis;           // 0 x (row), 1 y, 2 z
+  int gauge_axis;     // axis of the A operator applied by F_GAUGE_POST / gauge modes
+  unsigned flags;
+  double invN;        // normalisation folded into K (1/N) or A (1/n_axis)
+  const double* norm_in;
+  double* partials;
+  unsigned* counter;
+  double* result;     // norm_out (F_NORM) or kinetic sum (KSUM modes)
+  double dV;
+};
+
+// ---------------------------------------------------------------------------
+// Field evaluation (GPUE operators.cu kernels restated per point)
+// ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Equation programs: postfix form of GPUE's AST (src/dynamic.cu:527-603).
+// ---------------------------------------------------------------------------
+__device__ __noinline__ static double eval_eqn(const EqnProg* p, double x, double y, double z, double t) {
+  double st[EQ_MAX];
+  int sp = 0;
+  const int n = p->n;
+  for (int i = 0; i < n; ++i) {
+    const int op = p->op[i];
+    if (op == EQ_CONST) { st[sp++] = p->val[i]; continue; }
+    if (op == EQ_X) { st[sp++] = x; continue; }
+    if (op == EQ_Y) { st[sp++] = y; continue; }
+    if (op == EQ_Z) { st[sp++] = z; continue; }
+    if (op == EQ_T) { st[sp++] = t; continue; }
+    const double b = st[--sp], a = st[sp - 1];
+    double r = 0.0;
+    switch (op - EQ_OP) {
+      case 1: r = a + b; break;
+      case 2: r = a - b; break;
+      case 3: r = a * b; break;
+      case 4: r = a / b; break;
+      case 5: r = pow(a, (int)b); break;          // pow_gpu (:88-90)
+      case 6: r = cos(a); break;
+      case 7: r = sin(a); break;
+      case 8: r = tan(a); break;
+      case 9: r = sqrt(a); break;
+      case 10: r = exp(a); break;
+      case 11: { int k = (int)a; r = k == 0 ? j0(b) : k == 1 ? j1(b) : jn(k, b); break; }   // :138-148
+      case 12: { int k = (int)a; r = k == 0 ? y0(b) : k == 1 ? y1(b) : yn(k, b); break; }   // :150-160
+      default: r = 0.0; break;
+    }
+    st[sp - 1] = r;
+  }
+  return sp > 0 ? st[sp - 1] : 0.0;
+}
+
+__device__ __forceinline__ double A_val(const Phys& p, int axis, int xi, int yi, int zi, long gid) {
+  if (p.a_kind == A_ROTATION) {           // krotation_Ax/Ay, kconstant_A (:630-647, :621-627)
+    if (axis == 0) return -p.yg[yi] * p.omega * p.omegaX;
+    if (axis == 1) return p.xg[xi] * p.omega * p.omegaY;
+    return 0.0;
+  }
+  if (p.a_kind == A_ZERO) return 0.0;
+  return p.At[axis][gid];
+}
+
+__device__ __forceinline__ double V_val(const Phys& p, int xi, int yi, int zi, long gid) {
+  if (p.v_kind == V_HARMONIC) {           // kharmonic_V (:994-1008)
+    double V_x = p.omegaX * (p.xg[xi] + p.x0);
+    double V_y = p.gammaY * p.omegaY * (p.yg[yi] + p.y0);
+    double V_z = 1.0 * p.omegaZe * (p.zg[zi] + p.z0e);
+    double ax = A_val(p, 0, xi, yi, zi, gid), ay = A_val(p, 1, xi, yi, zi, gid),
+           az = A_val(p, 2, xi, yi, zi, gid);
+    return 0.5 * p.mass * ((V_x * V_x + V_y * V_y + V_z * V_z) + (ax * ax + ay * ay + az * az));
+  }
+  if (p.v_kind == V_TABLE) return p.Vt[gid];
+  // time-dependent equation field at find_field's coordinates (dynamic.cu:613-615)
+  return eval_eqn(p.veq, p.dxe * xi - p.xMax, p.dye * yi - p.yMax, p.dze * zi - p.zMax, p.time);
+}
+
+__device__ __forceinline__ double K_val(const Phys& p, int xi, int yi, int zi, long gid) {
+  if (p.k_kind == K_STANDARD)             // simple_K (:460-469)
+    return p.hb2m * (p.pxg[xi] * p.pxg[xi] + p.pyg[yi] * p.pyg[yi] + p.pzg[zi] * p.pzg[zi]);
+  return p.Kt[gid];
+}
+
+__device__ __forceinline__ long gid_of(const Phys& p, int xi, int yi, int zi) {
+  return (long)xi + (long)p.xDim * ((long)yi + (long)p.yDim * (long)zi);
+}
+
+// psi * exp(-arg) (imaginary time) or psi * exp(-i arg) (real time), times s
+template <typename T>
+__device__ __forceinline__ cx<T> apply_phase(cx<T> v, double arg, int gstate, double s) {
+  if (gstate == 0) {
+    T f = (T)(exp(-arg) * s);
+    return cscale(v, f);
+  }
+  double sn, cs;
+  sincos(-arg, &sn, &cs);
+  cx<T> w = mk((T)(cs * s), (T)(sn * s));
+  return cmul(w, v);
+}
+
+// V/2 with the nonlinear density term: cMultDensity (src/kernels.cu:365-388) with
+// GV/EV (src/operators.cu:1081,1100) merged into one exponential.
+template <typename T>
+__device__ __forceinline__ cx<T> op_halfV(const Phys& p, cx<T> v, int xi, int yi, int zi) {
+  long gid = gid_of(p, xi, yi, zi);
+  double V = V_val(p, xi, yi, zi, gid);
+  double arg = V * p.cV;
+  if (p.gpe) {
+    double d = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    arg += p.g * d * p.cH;
+  }
+  return apply_phase(v, arg, p.gstate, 1.0);
+}
+
+// K step: GK/EK (src/operators.cu:1082,1102) with both 1/sqrt(N) of src/evolution.cu:761,777
+template <typename T>
+__device__ __forceinline__ cx<T> op_K(const Phys& p, cx<T> v, int xi, int yi, int zi, double invN) {
+  long gid = gid_of(p, xi, yi, zi);
+  return apply_phase(v, K_val(p, xi, yi, zi, gid) * p.cK, p.gstate, invN);
+}
+
+// gauge operator along `axis`: GpA*/EpA* (src/operators.cu:1089-1110) with both 1/sqrt(n)
+template <typename T>
+__device__ __forceinline__ cx<T> op_A(const Phys& p, cx<T> v, int axis, int xi, int yi, int zi, double invn) {
+  long gid = gid_of(p, xi, yi, zi);
+  double k = axis == 0 ? p.pxg[xi] : axis == 1 ? p.pyg[yi] : p.pzg[zi];
+  double pA = A_val(p, axis, xi, yi, zi, gid) * k;
+  return apply_phase(v, pA * p.Dt, p.gstate, invn);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic grid reduction of NA doubles: per-workgroup partials, the last
+// workgroup (atomic ticket) sums them in a fixed order.
+// ---------------------------------------------------------------------------
+template <int NA>
+__device__ __forceinline__ void block_sum(double (&v)[NA], double* sh /* >= NA * 16 */) {
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    double x = v[a];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    v[a] = x;
+  }
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int a = 0; a < NA; ++a) sh[a * 16 + w] = v[a];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      double s = 0.0;
+      for (int i = 0; i < nw; ++i) s += sh[a * 16 + i];
+      v[a] = s;
+    }
+  }
+}
+
+// Returns true in thread 0 of the last block; `tot` then holds the grid totals.
+template <int NA>
+__device__ __forceinline__ bool grid_sum(double (&v)[NA], double* partials, unsigned* counter, double* sh,
+                                         double (&tot)[NA]) {
+  __shared__ int s_last;
+  block_sum<NA>(v, sh);
+  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int a = 0; a < NA; ++a) partials[(size_t)a * nb + b] = v[a];
+    __threadfence();
+    unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (t == nb - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  double acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    acc[a] = 0.0;
+    for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
+      acc[a] += __hip_atomic_load(partials + (size_t)a * nb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum<NA>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int a = 0; a < NA; ++a) tot[a] = acc[a];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return threadIdx.x == 0;
+}
+
+}  // namespace gpue

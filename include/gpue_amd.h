@@ -1,0 +1,150 @@
+/*
+ * gpue_amd.h -- C ABI of libgpue_amd.so, the MI355X (gfx950) split-operator
+ * step that stands in for GPUE's evolution/split_op path.
+ *
+ * Conventions
+ *  - Every call returns a gpue_status; the library never exits the process
+ *    (GPUE's cudaHandleError/cudaCheckError exit(1), src/split_op.cu:40-53).
+ *    gpue_last_error() gives the message of the last failing call on a ctx.
+ *  - Host buffers are caller-owned and only touched during the call.
+ *  - psi is complex, interleaved (re, im) doubles, index x + xDim*(y + yDim*z)
+ *    (x fastest), exactly GPUE's wfc layout (src/kernels.cu:28-35).
+ *  - One gpue_ctx is driven by one host thread at a time; contexts are
+ *    independent (each owns its device memory and one HIP stream).
+ *  - Device memory is allocated only by gpue_create / gpue_set_field* and freed
+ *    by gpue_destroy; the step loop allocates nothing.
+ */
+#ifndef GPUE_AMD_H
+#define GPUE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPUE_ABI_VERSION 1
+
+typedef struct gpue_ctx gpue_ctx;
+
+typedef enum {
+    GPUE_OK = 0,
+    GPUE_EINVAL = 1,        /* bad argument / parameter */
+    GPUE_ENOMEM = 2,        /* device allocation failed */
+    GPUE_EUNSUPPORTED = 3,  /* grid size or feature not implemented */
+    GPUE_EHIP = 4,          /* HIP runtime error (message in gpue_last_error) */
+    GPUE_ENONFINITE = 5,    /* norm or energy became NaN/Inf */
+    GPUE_EBUDGET = 6,       /* serving: request exceeds the HBM budget */
+    GPUE_EPARSE = 7         /* equation string rejected */
+} gpue_status;
+
+enum { GPUE_FIELD_V = 0, GPUE_FIELD_K = 1, GPUE_FIELD_AX = 2, GPUE_FIELD_AY = 3, GPUE_FIELD_AZ = 4 };
+enum { GPUE_A_ROTATION = 0, GPUE_A_CONSTANT = 1 };   /* GPUE -A rotation | constant (src/ds.cu:300-331) */
+
+/* GPUE's parameter set for the step (keys of Grid, include/ds.h:86-197, with the
+ * defaults of src/parser.cu:14-67).  hbar, mass and g are explicit so that
+ * harmonic-oscillator units can be used; gpue_params_default() fills GPUE's SI
+ * values (HBAR include/constants.h:20, Rb-87 mass src/init.cu:98). */
+typedef struct {
+    int dimnum, xDim, yDim, zDim;
+    double xMax, yMax, zMax;        /* box half-widths: box_size, or 6*Rxy*a0 (src/init.cu:124-137) */
+    double hbar, mass;
+    double g;                       /* interaction * gDenConst (src/evolution.cu:739) */
+    double omega, omegaX, omegaY, omegaZ, gammaY;
+    double x0_shift, y0_shift, z0_shift;
+    double gdt, dt;                 /* imaginary / real time step (src/parser.cu:21-22) */
+    int gpe;                        /* -s: nonlinear term on */
+    int corotating;                 /* -l: gauge (A.p) operators on */
+    int a_kind;                     /* GPUE_A_ROTATION | GPUE_A_CONSTANT (tables via gpue_set_field) */
+    int precision;                  /* 64: FP64 throughout; 32: FP32 bulk, FP64 reductions */
+    int device;                     /* HIP device ordinal (-C, src/parser.cu:168) */
+} gpue_params;
+
+typedef struct {
+    int steps_done;     /* steps executed (early stop included) */
+    int early_stop;     /* 1 if the imaginary-time energy criterion stopped the loop */
+    int n_energies;     /* energies evaluated (cadence hits) */
+    double energy;      /* last evaluated energy (GPUE formula), 0 if none */
+    double mu;          /* chemical potential at the same point */
+} gpue_evolve_result;
+
+/* GPUE Grid defaults (src/parser.cu:14-67) with dimnum/xDim/... as given there. */
+gpue_status gpue_params_default(gpue_params *p);
+
+/* Replaces init()'s field/operator construction and plan creation for one problem
+ * (src/init.cu:40-356, src/operators.cu:722-992).  Fields default to GPUE's
+ * kharmonic_V (incl. 1/2 m |A|^2), krotation_A / kconstant_A and simple_K. */
+gpue_status gpue_create(const gpue_params *p, gpue_ctx **out);
+void        gpue_destroy(gpue_ctx *c);
+const char *gpue_last_error(const gpue_ctx *c);
+
+/* Static field from a host table of N doubles (GPUE -A file / file_A, and the
+ * V/K/A arrays GPUE keeps in Grid: src/operators.cu:505-520). */
+gpue_status gpue_set_field(gpue_ctx *c, int which, const double *host_N);
+/* Field from a GPUE equation string (src/dynamic.cu:187-470).  Expressions
+ * without 't' are evaluated once on the device at find_field's coordinates
+ * (src/dynamic.cu:605-616); a V with 't' is compiled into the fused V/2 pass.
+ * `names`/`values` supply the Grid doubles the expression may reference. */
+gpue_status gpue_set_field_eqn(gpue_ctx *c, int which, const char *expr, const char *const *names,
+                               const double *values, int n_names);
+
+/* set_variables' H2D of psi (src/init.cu:422,470) / D2H (src/evolution.cu:430). */
+gpue_status gpue_upload_wfc(gpue_ctx *c, const double *interleaved);
+gpue_status gpue_download_wfc(gpue_ctx *c, double *interleaved);
+
+/* evolve() (src/evolution.cu:260-908) without host I/O: steps i = first_step ..
+ * first_step+nsteps-1 of GPUE's loop, gstate 0 = imaginary, 1 = real time.
+ * energy_every > 0 evaluates energy_calc after steps with i % energy_every == 0,
+ * storing up to max_energies values, and stops early in imaginary time when
+ * |E_old - E| < energy_thr * E_old (src/evolution.cu:872-903). */
+gpue_status gpue_evolve(gpue_ctx *c, int first_step, int nsteps, int gstate, int energy_every,
+                        double energy_thr, double *energies, int max_energies, gpue_evolve_result *res);
+
+/* parSum (src/split_op.cu:83-129): psi <- psi / sqrt(sum |psi|^2 dV). */
+gpue_status gpue_parsum(gpue_ctx *c);
+
+/* energy_calc (src/split_op.cu:208-375), plus the chemical potential (same sum
+ * with the interaction weighted g instead of g/2). */
+gpue_status gpue_energy(gpue_ctx *c, double *energy, double *mu);
+
+/* Tracker::findVortex (src/tracker.cu:102-161) on the current 2D psi:
+ * marker_out (xDim*xDim ints, overwritten) and the count of hits. */
+gpue_status gpue_vortex_marker_2d(gpue_ctx *c, double mask_radius, int *marker_out, int *found);
+/* findVortex + vortPos (src/tracker.cu:212-229): raster-ordered (i, j, winding). */
+gpue_status gpue_vortex_list_2d(gpue_ctx *c, double mask_radius, int *ijw, int max_vortices, int *n_vortices);
+
+/* x, y, z (axis 0..2) and px, py, pz (3..5) grids of generate_p_space
+ * (src/operators.cu:277-432). */
+gpue_status gpue_get_grid(gpue_ctx *c, int axis, double *out);
+
+/* Interaction change between phases (quench).  GPUE has one `interaction`
+ * (src/evolution.cu:739); this is the ABI extension the quench workload needs. */
+gpue_status gpue_set_g(gpue_ctx *c, double g);
+
+/* Wait for all work on the ctx stream. */
+gpue_status gpue_sync(gpue_ctx *c);
+
+/* --- measurement hooks (no reference counterpart: GPUE prints clock() deltas,
+ * src/evolution.cu:436-441) --- */
+/* When on, gpue_evolve records a HIP event pair around every sweep launch. */
+gpue_status gpue_set_timing(gpue_ctx *c, int on);
+/* Per-sweep-kind accumulated kernel time: kind index, name, launches, ms. */
+gpue_status gpue_timing_get(gpue_ctx *c, int kind, const char **name, long *launches, double *ms);
+/* Algorithmic HBM bytes of one step in the current configuration (psi sweeps). */
+gpue_status gpue_step_bytes(gpue_ctx *c, int gstate, double *bytes_per_step, double *sweeps_per_step);
+
+/* --- equation strings (host-side, src/dynamic.cu:187-522) --- */
+typedef struct {          /* EqnNode_gpu (include/ds.h:71-80) */
+    double val;
+    int is_dynamic;
+    char var;
+    int left, right;
+    int op_num;
+} gpue_eqn_node;
+
+gpue_status gpue_eqn_parse(const char *expr, const char *const *names, const double *values, int n_names,
+                           gpue_eqn_node *nodes, int max_nodes, int *n_nodes, int *time_dependent);
+double      gpue_eqn_eval(const gpue_eqn_node *nodes, int n_nodes, double x, double y, double z, double t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUE_AMD_H */
