@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""Split-operator step benchmark on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): the 2D rotating
+condensate, complex128 psi on 512^2 over [-12,12)^2, V = (x^2+y^2)/2 (+ GPUE's
+1/2 |A|^2), g = 1000, Omega = 0.9 applied as A = Omega(-y, x) through the
+per-axis gauge operators; by default the full job of 4000 imaginary-time steps
+at dt = 1e-3 followed by 10000 real-time steps at dt = 1e-4, from a seeded
+random-phase Gaussian.  A step is one split-operator time step.  Inputs are
+resident in HBM when the timed region starts.
+
+Also reported (extra): the 3D ground state on 256^3 (V = (x^2+y^2+4z^2)/2,
+g = 5000, dt = 5e-4) in FP64 and FP32-bulk/FP64-reductions, the FP32/FP64
+speedup and energy error, per-kernel HIP-event timings and the
+roofline of the dominant kernels, single-solve latency of the 256^2 ground
+state, and a CPU baseline (the FP64 oracle on the host cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def seeded_gaussian(shape, grids, seed, phase, noise):
+    rng = np.random.default_rng(seed)
+    mesh = np.meshgrid(*grids[::-1], indexing="ij")
+    r2 = sum(m ** 2 for m in mesh)
+    psi = np.exp(-r2 / 2) * (1 + noise * rng.uniform(-1, 1, shape))
+    if phase:
+        psi = psi * np.exp(1j * rng.uniform(0, 2 * np.pi, shape))
+    return psi.astype(np.complex128)
+
+
+CFG2 = dict(dimnum=2, xDim=512, yDim=512, xMax=12.0, yMax=12.0, g=1000.0, gpe=1, corotating=1, omega=0.9,
+            gdt=1e-3, dt=1e-4)
+CFG3 = dict(dimnum=3, xDim=256, yDim=256, zDim=256, xMax=8.0, yMax=8.0, zMax=8.0, g=5000.0, gpe=1, omegaZ=2.0,
+            gdt=5e-4)
+CFG1 = dict(dimnum=2, xDim=256, yDim=256, xMax=10.0, yMax=10.0, g=500.0, gpe=1, gdt=1e-3)
+
+
+def make_ctx(G, prec, cfg, seed=1, phase=False, noise=0.01):
+    ctx = G.Context(precision=prec, **cfg)
+    grids = [ctx.grid(a) for a in range(ctx.dimnum)]
+    psi = seeded_gaussian(ctx.shape, grids, seed, phase, noise)
+    dV = 1.0
+    for a in range(ctx.dimnum):
+        dV *= grids[a][1] - grids[a][0]
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * dV)
+    ctx.upload(psi)
+    return ctx, psi
+
+
+def run_schedule(ctx, imag, real):
+    if imag:
+        ctx.evolve(imag, 0)
+    if real:
+        ctx.evolve(real, 1)
+
+
+def kernel_table(timing, bytes_per_launch):
+    rows = []
+    for name, (n, ms) in timing.items():
+        avg_us = ms / n * 1e3
+        rows.append(dict(kernel=name, launches=n, total_ms=ms, avg_us=avg_us,
+                         gbs=bytes_per_launch / (avg_us * 1e-6) / 1e9))
+    rows.sort(key=lambda r: -r["total_ms"])
+    return rows
+
+
+def roofline(rows, bytes_per_launch):
+    top = rows[0]
+    achieved = bytes_per_launch / (top["avg_us"] * 1e-6) / 1e9
+    return {"bound": "hbm", "kernel": top["kernel"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(top["avg_us"], 3)}
+
+
+def cpu_baseline(budget_s):
+    from oracle import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    orc = O.Oracle(**CFG2)
+    grids = [orc.grid(0), orc.grid(1)]
+    psi = seeded_gaussian((512, 512), grids, 1, True, 0.0)
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
+    orc.set_wfc(psi)
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s or steps < 20:
+        for i in range(10):
+            orc.step(steps + i, 0 if steps < 4000 else 1)
+        steps += 10
+    el = time.perf_counter() - t0
+    return {"value": round(steps / el, 3), "unit": "steps/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
+            "kind": "port",
+            "sample": f"FP64 oracle (oracle/gpue_oracle.c, OpenMP) on the same 512^2 gauge workload: first "
+                      f"{steps} imaginary-time steps of the job, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=14000, help="timed steps (first 4000/14000 imaginary, rest real)")
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--no-extra", action="store_true", help="skip the 256^3 / latency extras")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--only", choices=["512", "256c"], default=None, help="profiling: run one workload only")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; every solve runs on a single MI355X "
+              "(multi-GPU sharding is a design note only). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import gpue_amd as G
+
+    K, W = args.steps, args.warmup
+    imag = int(round(K * 4000 / 14000))
+    real = K - imag
+    extra = {}
+
+    # ---------------- headline: 512^2 rotating condensate, FP64, gauge ----------------
+    if args.only in (None, "512"):
+        ctx, psi0 = make_ctx(G, 64, CFG2, phase=True, noise=0.0)
+        run_schedule(ctx, W // 2, W - W // 2)
+        ctx.upload(psi0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        run_schedule(ctx, imag, real)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        sps = K / el
+        e_final, mu_final = ctx.energy()
+        bps_imag, P_imag = ctx.step_bytes(0)
+        bps_real, P_real = ctx.step_bytes(1)
+        bytes_job = imag * bps_imag + real * bps_real
+        # second pass with a HIP event pair around every sweep launch (ctx stream)
+        ctx.upload(psi0)
+        ctx.set_timing(True)
+        ctx.sync()
+        t1 = time.perf_counter()
+        run_schedule(ctx, imag, real)
+        ctx.sync()
+        el_ev = time.perf_counter() - t1
+        rows = kernel_table(ctx.timing(), 2 * 16 * 512 * 512)
+        ctx.set_timing(False)
+        ctx.close()
+        log(f"512^2 FP64 gauge: {sps:.1f} steps/s ({el / K * 1e6:.2f} us/step), E={e_final:.10g}")
+        for r in rows:
+            log(f"   {r['kernel']:30s} n={r['launches']:6d} avg={r['avg_us']:8.2f} us  {r['gbs']:8.1f} GB/s")
+        roof = roofline(rows, 2 * 16 * 512 * 512)
+        extra["512sq_fp64_gauge"] = {
+            "steps_per_s": round(sps, 1), "us_per_step": round(el / K * 1e6, 3),
+            "effective_GBps": round(bytes_job / el / 1e9, 1), "sweeps_per_step_imag": P_imag,
+            "sweeps_per_step_real": P_real, "energy_final": e_final, "mu_final": mu_final,
+            "events_pass_steps_per_s": round(K / el_ev, 1), "kernels": rows}
+    else:
+        sps, el, roof = float("nan"), float("nan"), None
+
+    # ---------------- 256^3 FP64 vs FP32 ----------------
+    if not args.no_extra and args.only in (None, "256c"):
+        K3 = 200
+        res3 = {}
+        for prec in (64, 32):
+            ctx, psi0 = make_ctx(G, prec, CFG3, seed=1)
+            ctx.evolve(10, 0)
+            ctx.upload(psi0)
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.evolve(K3, 0)
+            ctx.sync()
+            el3 = time.perf_counter() - t0
+            e3, mu3 = ctx.energy()
+            b3, P3 = ctx.step_bytes(0)
+            ctx.upload(psi0)
+            ctx.set_timing(True)
+            ctx.evolve(50, 0)
+            ctx.sync()
+            rows3 = kernel_table(ctx.timing(), 2 * (16 if prec == 64 else 8) * 256 ** 3)
+            ctx.set_timing(False)
+            ctx.close()
+            res3[prec] = dict(steps_per_s=round(K3 / el3, 2), us_per_step=round(el3 / K3 * 1e6, 2),
+                              effective_GBps=round(b3 * K3 / el3 / 1e9, 1),
+                              frac_of_8TBps=round(b3 * K3 / el3 / 1e9 / HBM_PEAK_GBS, 4), sweeps_per_step=P3,
+                              energy=e3, mu=mu3, kernels=rows3,
+                              roofline=roofline(rows3, 2 * (16 if prec == 64 else 8) * 256 ** 3))
+            log(f"256^3 FP{prec}: {K3 / el3:.1f} steps/s, {b3 * K3 / el3 / 1e9:.0f} GB/s effective, E={e3:.12g}")
+            for r in rows3:
+                log(f"   {r['kernel']:30s} n={r['launches']:6d} avg={r['avg_us']:8.2f} us  {r['gbs']:8.1f} GB/s")
+        extra["256cube_fp64"] = res3[64]
+        extra["256cube_fp32"] = res3[32]
+        extra["fp32_over_fp64_speedup_256cube"] = round(res3[32]["steps_per_s"] / res3[64]["steps_per_s"], 3)
+        extra["fp32_rel_energy_error_256cube"] = abs(res3[32]["energy"] - res3[64]["energy"]) / abs(res3[64]["energy"])
+        extra["fp32_rel_mu_error_256cube"] = abs(res3[32]["mu"] - res3[64]["mu"]) / abs(res3[64]["mu"])
+
+        # single-solve latency of the 256^2 ground state (config 1: 2000 imaginary steps)
+        lat = []
+        for rep in range(7):
+            ctx, _ = make_ctx(G, 64, CFG1, seed=1)
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.evolve(2000, 0)
+            ctx.energy()
+            ctx.sync()
+            lat.append(time.perf_counter() - t0)
+            ctx.close()
+        lat = np.array(lat[1:]) * 1e3
+        extra["solve_latency_256sq_2000steps_ms"] = {"p50": round(float(np.percentile(lat, 50)), 3),
+                                                     "p99": round(float(np.percentile(lat, 99)), 3),
+                                                     "samples": len(lat)}
+
+    cpu = None
+    if not args.no_cpu and args.only is None:
+        cpu = cpu_baseline(args.cpu_seconds)
+        log(f"CPU baseline: {cpu['value']} steps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": "split-op steps/s (512^2 rotating condensate, FP64, gauge Omega=0.9)",
+        "value": round(sps, 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(el / K * 1e3, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random-phase Gaussian; fields generated in-kernel)",
+        "config": {"workload": "2D rotating condensate 512^2 over [-12,12)^2, g=1000, Omega=0.9 gauge, "
+                               f"{imag} imaginary (dt=1e-3) + {real} real (dt=1e-4) steps",
+                   "grid": [512, 512], "precision": "fp64", "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -56,14 +56,17 @@ struct gpue_ctx {
   std::map<int, void*> tw;   // N -> twiddle table in the working precision
   double* partials = nullptr;
   size_t partials_n = 0;
-  unsigned* counter = nullptr;
   double* results = nullptr;   // [0] norm  [1] kinetic  [2..5] position terms
-  int v_kind = V_HARMONIC, a_kind = A_ROTATION, k_kind = K_STANDARD;
+  int v_kind = V_SEP, a_kind = A_ROTATION, k_kind = K_SEP;
+  bool v_user = false;         // V supplied by the caller (table or equation)
+  double* vs[3] = {};          // separable V / K value tables
+  double* ks[3] = {};
   double* Vt = nullptr;
   double* Kt = nullptr;
   double* At[3] = {};
   bool az_zero = true;
-  EqnProg* veq = nullptr;
+  bool v_time = false;         // V is a time-dependent equation, refreshed each step
+  EqnProg veq;
   bool pending_norm = false;
   double last_energy = 0.0;
   std::string err;
@@ -135,21 +138,16 @@ static Phys make_phys(const gpue_ctx* c, int gstate, double time) {
   f.x0 = p.x0_shift;
   f.y0 = p.y0_shift;
   f.z0e = p.dimnum == 3 ? p.z0_shift : 0.0;     // items[8]
-  f.time = time;
-  f.dxe = c->d[0];
-  f.dye = c->d[1];
-  f.dze = c->d[2];
-  f.xMax = p.xMax;
-  f.yMax = p.dimnum > 1 ? p.yMax : 0.0;
-  f.zMax = p.dimnum > 2 ? p.zMax : 0.0;
+  (void)time;
   for (int a = 0; a < 3; ++a) {
     (&f.xg)[a] = c->dgrid[a];
     (&f.pxg)[a] = c->dgrid[a + 3];
+    f.vs[a] = c->vs[a];
+    f.ks[a] = c->ks[a];
   }
   f.Vt = c->Vt;
   f.Kt = c->Kt;
   for (int a = 0; a < 3; ++a) f.At[a] = c->At[a];
-  f.veq = c->veq;
   return f;
 }
 
@@ -239,9 +237,6 @@ static gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const voi
   a.invN = s.invN;
   a.norm_in = c->results + 0;
   a.partials = c->partials;
-  a.counter = c->counter;
-  a.result = (s.mode == (s.row ? (int)RM_KSUM : (int)CM_KSUM)) ? c->results + 1 : c->results + 0;
-  a.dV = c->d[0] * c->d[1] * c->d[2];
   int slot = -1;
   if (c->timing) {
     if (c->ev_used == TIMING_PAIRS) {
@@ -257,6 +252,16 @@ static gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const voi
   if (!handled) return fail(c, GPUE_EUNSUPPORTED, "no sweep kernel for n=" + std::to_string(n));
   if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("sweep launch: ") + hipGetErrorString(e));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2 * slot + 1], c->stream));
+  // second stage of the fused reductions
+  const bool ksum = s.mode == (s.row ? (int)RM_KSUM : (int)CM_KSUM);
+  if (ksum || (s.flags & F_NORM)) {
+    const int nb = s.row ? sweep_blocks(true, n, c->dims[1] * c->dims[2], 1)
+                         : sweep_blocks(false, n, c->dims[0], s.axis == 1 ? c->dims[2] : c->dims[1]);
+    const double dV = c->d[0] * c->d[1] * c->d[2];
+    e = ksum ? launch_finalize(c->partials, nb, 1, 0, s.invN, c->results + 1, c->stream)
+             : launch_finalize(c->partials, nb, 1, 1, dV, c->results + 0, c->stream);
+    if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("finalize: ") + hipGetErrorString(e));
+  }
   return GPUE_OK;
 }
 
@@ -270,6 +275,54 @@ static gpue_status apply_pending(gpue_ctx* c) {
 }
 
 static bool valid_dim(int n) { return sweep_supported(n); }
+
+// Per-axis terms of kharmonic_V (src/operators.cu:994-1008) with the rotation
+// gauge's 1/2 m |A|^2 (Ax = -y omega omegaX, Ay = x omega omegaY, Az = 0,
+// src/operators.cu:621-647) and of simple_K (src/operators.cu:460-469).
+static void sep_tables(const gpue_ctx* c, std::vector<double>* vs, std::vector<double>* ks) {
+  const gpue_params& p = c->p;
+  const double m = p.mass, hb2m = p.hbar * p.hbar / (2 * p.mass);
+  const bool rot = c->a_kind == A_ROTATION;
+  const double oZ = p.dimnum == 3 ? p.omegaZ : 0.0, z0 = p.dimnum == 3 ? p.z0_shift : 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const std::vector<double>& x = c->grid[a];
+    const std::vector<double>& k = c->grid[a + 3];
+    vs[a].resize(x.size());
+    ks[a].resize(k.size());
+    for (size_t i = 0; i < x.size(); ++i) {
+      double t, A = 0.0;
+      if (a == 0) { t = p.omegaX * (x[i] + p.x0_shift); if (rot) A = x[i] * p.omega * p.omegaY; }
+      else if (a == 1) { t = p.gammaY * p.omegaY * (x[i] + p.y0_shift); if (rot) A = -x[i] * p.omega * p.omegaX; }
+      else { t = 1.0 * oZ * (x[i] + z0); }
+      vs[a][i] = 0.5 * m * (t * t + A * A);
+      ks[a][i] = hb2m * (k[i] * k[i]);
+    }
+  }
+}
+
+// A full V table in GPUE's kharmonic_V form for the current A tables.
+static gpue_status materialise_V(gpue_ctx* c) {
+  const gpue_params& p = c->p;
+  std::vector<double> A[3];
+  for (int a = 0; a < 3; ++a) {
+    A[a].assign(c->N, 0.0);
+    if (c->At[a]) HIPCHK(c, hipMemcpy(A[a].data(), c->At[a], c->N * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  const double oZ = p.dimnum == 3 ? p.omegaZ : 0.0, z0 = p.dimnum == 3 ? p.z0_shift : 0.0;
+  std::vector<double> V(c->N);
+  for (long gid = 0; gid < c->N; ++gid) {
+    long xi = gid % c->dims[0], yi = (gid / c->dims[0]) % c->dims[1], zi = gid / ((long)c->dims[0] * c->dims[1]);
+    double Vx = p.omegaX * (c->grid[0][xi] + p.x0_shift);
+    double Vy = p.gammaY * p.omegaY * (c->grid[1][yi] + p.y0_shift);
+    double Vz = 1.0 * oZ * (c->grid[2][zi] + z0);
+    V[gid] = 0.5 * p.mass * ((Vx * Vx + Vy * Vy + Vz * Vz) + (A[0][gid] * A[0][gid] + A[1][gid] * A[1][gid] +
+                                                            A[2][gid] * A[2][gid]));
+  }
+  if (!c->Vt && hipMalloc((void**)&c->Vt, c->N * sizeof(double)) != hipSuccess) return fail(c, GPUE_ENOMEM, "V table");
+  HIPCHK(c, hipMemcpy(c->Vt, V.data(), c->N * sizeof(double), hipMemcpyHostToDevice));
+  c->v_kind = V_TABLE;
+  return GPUE_OK;
+}
 
 // ---------------------------------------------------------------------------
 // ABI
@@ -315,8 +368,9 @@ const char* gpue_last_error(const gpue_ctx* c) { return c ? c->err.c_str() : "nu
 void gpue_destroy(gpue_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* bufs[] = {c->psi, c->tmp, c->partials, c->counter, c->results, c->Vt, c->Kt, c->At[0], c->At[1],
-                  c->At[2], c->veq, c->vmask, c->vcand, c->vmarker, c->vfound};
+  void* bufs[] = {c->psi, c->tmp, c->partials, c->results, c->Vt, c->Kt, c->At[0], c->At[1], c->At[2],
+                  c->vs[0], c->vs[1], c->vs[2], c->ks[0], c->ks[1], c->ks[2],
+                  c->vmask, c->vcand, c->vmarker, c->vfound};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int a = 0; a < 6; ++a)
@@ -361,6 +415,8 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
   c->N = (long)p.xDim * p.yDim * p.zDim;
   c->a_kind = p.a_kind == GPUE_A_ROTATION ? A_ROTATION : A_ZERO;
   make_grids(p.dimnum, p.xDim, p.yDim, p.zDim, p.xMax, p.yMax, p.zMax, c->grid, c->d);
+  std::vector<double> vsh[3], ksh[3];
+  sep_tables(c, vsh, ksh);
 
   hipError_t e = hipSetDevice(p.device);
   if (e != hipSuccess) return bail(GPUE_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
@@ -371,6 +427,12 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
   for (int a = 0; a < 6; ++a) {
     c->dgrid[a] = dev_copy(c->grid[a], &e);
     if (e != hipSuccess) return bail(GPUE_ENOMEM, "grid arrays");
+  }
+  for (int a = 0; a < 3; ++a) {
+    c->vs[a] = dev_copy(vsh[a], &e);
+    if (e != hipSuccess) return bail(GPUE_ENOMEM, "V tables");
+    c->ks[a] = dev_copy(ksh[a], &e);
+    if (e != hipSuccess) return bail(GPUE_ENOMEM, "K tables");
   }
   for (int a = 0; a < p.dimnum; ++a) {
     int n = c->dims[a];
@@ -398,9 +460,9 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
   if (p.dimnum > 2) nb = std::max<size_t>(nb, sweep_blocks(false, p.zDim, p.xDim, p.yDim));
   c->partials_n = nb * 4;
   if (hipMalloc((void**)&c->partials, c->partials_n * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&c->counter, 64) != hipSuccess || hipMalloc((void**)&c->results, 16 * sizeof(double)) != hipSuccess)
+      hipMalloc((void**)&c->results, 16 * sizeof(double)) != hipSuccess)
     return bail(GPUE_ENOMEM, "reduction scratch");
-  if (hipMemset(c->counter, 0, 64) != hipSuccess || hipMemset(c->results, 0, 16 * sizeof(double)) != hipSuccess ||
+  if (hipMemset(c->results, 0, 16 * sizeof(double)) != hipSuccess ||
       hipMemset(c->psi, 0, c->N * c->esz) != hipSuccess)
     return bail(GPUE_EHIP, "memset");
   c->ev.resize(2 * TIMING_PAIRS);
@@ -438,9 +500,13 @@ static gpue_status set_table(gpue_ctx* c, int which, const double* src, bool fro
   HIPCHK(c, hipMemcpyAsync(*slot, src, c->N * sizeof(double), from_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (which == GPUE_FIELD_V) c->v_kind = V_TABLE;
-  else if (which == GPUE_FIELD_K) c->k_kind = K_TABLE;
-  else {
+  if (which == GPUE_FIELD_V) {
+    c->v_kind = V_TABLE;
+    c->v_user = true;
+    c->v_time = false;
+  } else if (which == GPUE_FIELD_K) {
+    c->k_kind = K_TABLE;
+  } else {
     // the other components keep their current values: materialise them as tables
     if (c->a_kind != A_TABLE) {
       for (int a = 0; a < 3; ++a) {
@@ -464,6 +530,8 @@ static gpue_status set_table(gpue_ctx* c, int which, const double* src, bool fro
       HIPCHK(c, hipMemcpy(h.data(), c->At[2], c->N * sizeof(double), hipMemcpyDeviceToHost));
       c->az_zero = std::all_of(h.begin(), h.end(), [](double v) { return v == 0.0; });
     }
+    // GPUE builds V after A (src/operators.cu:726-829): a harmonic V follows the new A
+    if (!c->v_user) return materialise_V(c);
   }
   return GPUE_OK;
 }
@@ -497,9 +565,11 @@ gpue_status gpue_set_field_eqn(gpue_ctx* c, int which, const char* expr, const c
   if (td) {
     if (which != GPUE_FIELD_V)
       return fail(c, GPUE_EUNSUPPORTED, "time-dependent K and A fields are not supported (V only)");
-    if (!c->veq && hipMalloc((void**)&c->veq, sizeof(EqnProg)) != hipSuccess) return fail(c, GPUE_ENOMEM, "eqn");
-    HIPCHK(c, hipMemcpy(c->veq, &dp, sizeof(EqnProg), hipMemcpyHostToDevice));
-    c->v_kind = V_EQN;
+    if (!c->Vt && hipMalloc((void**)&c->Vt, c->N * sizeof(double)) != hipSuccess) return fail(c, GPUE_ENOMEM, "V");
+    c->veq = dp;
+    c->v_kind = V_TABLE;
+    c->v_user = true;
+    c->v_time = true;
     return GPUE_OK;
   }
   // static: evaluate once on the device (find_field), then use as a table
@@ -551,9 +621,10 @@ gpue_status gpue_parsum(gpue_ctx* c) {
   gpue_status st = apply_pending(c);
   if (st != GPUE_OK) return st;
   const double dV = c->d[0] * c->d[1] * c->d[2];
-  hipError_t e = c->prec == 64
-                     ? launch_norm_sum<double>(c->psi, c->N, dV, c->partials, c->counter, c->results, c->stream)
-                     : launch_norm_sum<float>(c->psi, c->N, dV, c->partials, c->counter, c->results, c->stream);
+  int nb = 0;
+  hipError_t e = c->prec == 64 ? launch_norm_sum<double>(c->psi, c->N, c->partials, &nb, c->stream)
+                               : launch_norm_sum<float>(c->psi, c->N, c->partials, &nb, c->stream);
+  if (e == hipSuccess) e = launch_finalize(c->partials, nb, 1, 1, dV, c->results, c->stream);
   if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
   c->pending_norm = true;
   return apply_pending(c);
@@ -576,10 +647,12 @@ gpue_status gpue_energy(gpue_ctx* c, double* energy, double* mu) {
   }
   if (st != GPUE_OK) return st;
   const int gauge = p.corotating && p.dimnum > 1;
+  int nb = 0;
   hipError_t e = c->prec == 64 ? launch_energy_pos<double>(c->psi, ph, gauge, p.hbar, c->d[0], c->d[1], c->d[2],
-                                                           c->partials, c->counter, c->results + 2, c->stream)
+                                                           c->partials, &nb, c->stream)
                                : launch_energy_pos<float>(c->psi, ph, gauge, p.hbar, c->d[0], c->d[1], c->d[2],
-                                                          c->partials, c->counter, c->results + 2, c->stream);
+                                                          c->partials, &nb, c->stream);
+  if (e == hipSuccess) e = launch_finalize(c->partials, nb, 4, 0, 1.0, c->results + 2, c->stream);
   if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
   double r[6];
   HIPCHK(c, hipMemcpyAsync(r, c->results, sizeof(r), hipMemcpyDeviceToHost, c->stream));
@@ -594,6 +667,16 @@ gpue_status gpue_energy(gpue_ctx* c, double* energy, double* mu) {
   return GPUE_OK;
 }
 
+// A time-dependent V is evaluated at time Dt*i over the grid before step i
+// (both half steps of GPUE's step use that time, src/evolution.cu:732,785).
+static gpue_status refresh_V(gpue_ctx* c, double t) {
+  const gpue_params& p = c->p;
+  hipError_t e = launch_eqn_field(c->veq, c->Vt, c->dims[0], c->dims[1], c->dims[2], c->d[0], c->d[1], c->d[2], p.xMax,
+                                  p.dimnum > 1 ? p.yMax : 0.0, p.dimnum > 2 ? p.zMax : 0.0, t, c->stream);
+  if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
+  return GPUE_OK;
+}
+
 gpue_status gpue_evolve(gpue_ctx* c, int first_step, int nsteps, int gstate, int energy_every, double energy_thr,
                         double* energies, int max_energies, gpue_evolve_result* res) {
   if (!c || nsteps < 0 || (gstate != 0 && gstate != 1) || first_step < 0) return GPUE_EINVAL;
@@ -605,6 +688,10 @@ gpue_status gpue_evolve(gpue_ctx* c, int first_step, int nsteps, int gstate, int
   for (s = 0; s < nsteps; ++s) {
     const int i = first_step + s;
     Phys ph = make_phys(c, gstate, Dt * i);   // time = Dt*i (src/evolution.cu:408)
+    if (c->v_time) {
+      gpue_status st = refresh_V(c, Dt * i);
+      if (st != GPUE_OK) return st;
+    }
     schedule(c, gstate, i, c->pending_norm, sched);
     for (const Sweep& sw : sched) {
       gpue_status st = launch(c, sw, ph, c->psi, c->psi);

@@ -6,16 +6,17 @@ namespace gpue {
 
 constexpr int REDUCE_BLOCKS = 1024;   // grid of the standalone reductions
 
+struct EqnProg;
 hipError_t launch_eqn_field(const EqnProg& prog, double* out, int xDim, int yDim, int zDim, double dx, double dy,
                             double dz, double xMax, double yMax, double zMax, double t, hipStream_t s);
 template <typename T>
 hipError_t launch_apply_norm(void* psi, long N, const double* norm, hipStream_t s);
 template <typename T>
-hipError_t launch_norm_sum(const void* psi, long N, double dV, double* partials, unsigned* counter, double* norm_out,
-                           hipStream_t s);
+hipError_t launch_norm_sum(const void* psi, long N, double* partials, int* nb, hipStream_t s);
+hipError_t launch_finalize(const double* partials, int nb, int na, int mode, double scale, double* out, hipStream_t s);
 template <typename T>
 hipError_t launch_energy_pos(const void* psi, const Phys& p, int gauge, double hbar, double dx, double dy, double dz,
-                             double* partials, unsigned* counter, double* out, hipStream_t s);
+                             double* partials, int* nb, hipStream_t s);
 template <typename T>
 hipError_t launch_vortex(const void* psi, const unsigned char* mask, int xDim, int* cand, int* marker, int* found,
                          hipStream_t s);

@@ -7,6 +7,42 @@
 
 namespace gpue {
 
+// ---------------------------------------------------------------------------
+// Equation programs: postfix form of GPUE's AST (src/dynamic.cu:527-603).
+// ---------------------------------------------------------------------------
+__device__ static double eval_eqn(const EqnProg* p, double x, double y, double z, double t) {
+  double st[EQ_MAX];
+  int sp = 0;
+  const int n = p->n;
+  for (int i = 0; i < n; ++i) {
+    const int op = p->op[i];
+    if (op == EQ_CONST) { st[sp++] = p->val[i]; continue; }
+    if (op == EQ_X) { st[sp++] = x; continue; }
+    if (op == EQ_Y) { st[sp++] = y; continue; }
+    if (op == EQ_Z) { st[sp++] = z; continue; }
+    if (op == EQ_T) { st[sp++] = t; continue; }
+    const double b = st[--sp], a = st[sp - 1];
+    double r = 0.0;
+    switch (op - EQ_OP) {
+      case 1: r = a + b; break;
+      case 2: r = a - b; break;
+      case 3: r = a * b; break;
+      case 4: r = a / b; break;
+      case 5: r = pow(a, (int)b); break;          // pow_gpu (:88-90)
+      case 6: r = cos(a); break;
+      case 7: r = sin(a); break;
+      case 8: r = tan(a); break;
+      case 9: r = sqrt(a); break;
+      case 10: r = exp(a); break;
+      case 11: { int k = (int)a; r = k == 0 ? j0(b) : k == 1 ? j1(b) : jn(k, b); break; }   // :138-148
+      case 12: { int k = (int)a; r = k == 0 ? y0(b) : k == 1 ? y1(b) : yn(k, b); break; }   // :150-160
+      default: r = 0.0; break;
+    }
+    st[sp - 1] = r;
+  }
+  return sp > 0 ? st[sp - 1] : 0.0;
+}
+
 // find_field (src/dynamic.cu:605-616): static field at dx*xid - xMax, ...
 __global__ void k_eqn_field(EqnProg prog, double* out, int xDim, int yDim, int zDim, double dx, double dy,
                             double dz, double xMax, double yMax, double zMax, double t) {
@@ -33,16 +69,30 @@ __global__ void k_apply_norm(cx<T>* psi, long N, const double* norm) {
 
 // parSum on its own (src/split_op.cu:83-129): sum |psi|^2 then divide
 template <typename T>
-__global__ void k_norm_sum(const cx<T>* psi, long N, double dV, double* partials, unsigned* counter,
-                           double* norm_out) {
+__global__ void k_norm_sum(const cx<T>* psi, long N, double* partials) {
   __shared__ double sh[64];
   double acc = 0.0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
     cx<T> v = psi[i];
     acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
   }
-  double pv[1] = {acc}, tot[1];
-  if (grid_sum<1>(pv, partials, counter, sh, tot)) norm_out[0] = sqrt(tot[0] * dV);
+  double pv[1] = {acc};
+  block_partial<1>(pv, partials, sh);
+}
+
+// Second stage of every reduction: one workgroup sums nb partials of each of
+// na accumulators in a fixed order.  mode 0: out[a] = tot[a] * scale;
+// mode 1: out[0] = sqrt(tot[0] * scale)  (the parSum norm, src/kernels.cu:482).
+__global__ void __launch_bounds__(256) k_finalize(const double* partials, int nb, int na, int mode, double scale,
+                                                  double* out) {
+  __shared__ double sh[64];
+  for (int a = 0; a < na; ++a) {
+    double acc[1] = {0.0};
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc[0] += partials[(size_t)a * nb + i];
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) out[a] = mode == 1 ? sqrt(acc[0] * scale) : acc[0] * scale;
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -54,7 +104,7 @@ __global__ void k_norm_sum(const cx<T>* psi, long N, double dV, double* partials
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void k_energy_pos(const cx<T>* psi, Phys p, int gauge, double hbar, double dx, double dy,
-                             double dz, double* partials, unsigned* counter, double* out) {
+                             double dz, double* partials) {
   __shared__ double sh[64];
   const long N = (long)p.xDim * p.yDim * p.zDim;
   const long sy = p.xDim, sz = (long)p.xDim * p.yDim;
@@ -92,10 +142,7 @@ __global__ void k_energy_pos(const cx<T>* psi, Phys p, int gauge, double hbar, d
       acc[2] += wx * (-syy * hbar) - (-wy) * (sx * hbar);
     }
   }
-  double tot[4];
-  if (grid_sum<4>(acc, partials, counter, sh, tot)) {
-    out[0] = tot[0]; out[1] = tot[1]; out[2] = tot[2]; out[3] = tot[3];
-  }
+  block_partial<4>(acc, partials, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -204,19 +251,24 @@ hipError_t launch_apply_norm(void* psi, long N, const double* norm, hipStream_t 
 }
 
 template <typename T>
-hipError_t launch_norm_sum(const void* psi, long N, double dV, double* partials, unsigned* counter, double* norm_out,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(k_norm_sum<T>, dim3(gs_blocks(N, 256, REDUCE_BLOCKS)), dim3(256), 0, s, (const cx<T>*)psi, N, dV,
-                     partials, counter, norm_out);
+hipError_t launch_norm_sum(const void* psi, long N, double* partials, int* nb, hipStream_t s) {
+  *nb = gs_blocks(N, 256, REDUCE_BLOCKS);
+  hipLaunchKernelGGL(k_norm_sum<T>, dim3(*nb), dim3(256), 0, s, (const cx<T>*)psi, N, partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const double* partials, int nb, int na, int mode, double scale, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nb, na, mode, scale, out);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_energy_pos(const void* psi, const Phys& p, int gauge, double hbar, double dx, double dy, double dz,
-                             double* partials, unsigned* counter, double* out, hipStream_t s) {
+                             double* partials, int* nb, hipStream_t s) {
   long N = (long)p.xDim * p.yDim * p.zDim;
-  hipLaunchKernelGGL(k_energy_pos<T>, dim3(gs_blocks(N, 256, REDUCE_BLOCKS)), dim3(256), 0, s, (const cx<T>*)psi, p,
-                     gauge, hbar, dx, dy, dz, partials, counter, out);
+  *nb = gs_blocks(N, 256, REDUCE_BLOCKS);
+  hipLaunchKernelGGL(k_energy_pos<T>, dim3(*nb), dim3(256), 0, s, (const cx<T>*)psi, p, gauge, hbar, dx, dy, dz,
+                     partials);
   return hipGetLastError();
 }
 
@@ -235,12 +287,12 @@ hipError_t launch_vortex(const void* psi, const unsigned char* mask, int xDim, i
 
 template hipError_t launch_apply_norm<double>(void*, long, const double*, hipStream_t);
 template hipError_t launch_apply_norm<float>(void*, long, const double*, hipStream_t);
-template hipError_t launch_norm_sum<double>(const void*, long, double, double*, unsigned*, double*, hipStream_t);
-template hipError_t launch_norm_sum<float>(const void*, long, double, double*, unsigned*, double*, hipStream_t);
+template hipError_t launch_norm_sum<double>(const void*, long, double*, int*, hipStream_t);
+template hipError_t launch_norm_sum<float>(const void*, long, double*, int*, hipStream_t);
 template hipError_t launch_energy_pos<double>(const void*, const Phys&, int, double, double, double, double, double*,
-                                              unsigned*, double*, hipStream_t);
+                                              int*, hipStream_t);
 template hipError_t launch_energy_pos<float>(const void*, const Phys&, int, double, double, double, double, double*,
-                                             unsigned*, double*, hipStream_t);
+                                             int*, hipStream_t);
 template hipError_t launch_vortex<double>(const void*, const unsigned char*, int, int*, int*, int*, hipStream_t);
 template hipError_t launch_vortex<float>(const void*, const unsigned char*, int, int*, int*, int*, hipStream_t);
 

@@ -132,11 +132,8 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row_sweep(SweepArgs a) {
     }
   }
   if (MODE == RM_KSUM || (flags & F_NORM)) {
-    double pv[1] = {acc}, tot[1];
-    if (grid_sum<1>(pv, a.partials, a.counter, sh, tot)) {
-      if constexpr (MODE == RM_KSUM) a.result[0] = tot[0] * a.invN;
-      else a.result[0] = sqrt(tot[0] * a.dV);
-    }
+    double pv[1] = {acc};
+    block_partial<1>(pv, a.partials, sh);
   }
 }
 
@@ -221,11 +218,8 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col_sweep(SweepArgs a) {
     }
   }
   if (MODE == CM_KSUM || (flags & F_NORM)) {
-    double pv[1] = {acc}, tot[1];
-    if (grid_sum<1>(pv, a.partials, a.counter, sh, tot)) {
-      if constexpr (MODE == CM_KSUM) a.result[0] = tot[0] * a.invN;
-      else a.result[0] = sqrt(tot[0] * a.dV);
-    }
+    double pv[1] = {acc};
+    block_partial<1>(pv, a.partials, sh);
   }
 }
 

@@ -18,9 +18,11 @@
 
 namespace gpue {
 
-enum { V_HARMONIC = 0, V_TABLE = 1, V_EQN = 2 };
+// V: separable 1-D value tables (harmonic trap + 1/2 m |A|^2 of the rotation gauge)
+// or a full table (file / equation fields, V re-evaluated per step when time dependent).
+enum { V_SEP = 0, V_TABLE = 1 };
 enum { A_ROTATION = 0, A_ZERO = 1, A_TABLE = 2 };
-enum { K_STANDARD = 0, K_TABLE = 1 };
+enum { K_SEP = 0, K_TABLE = 1 };
 
 // Sweep modes: row (x) and strided (y/z) kernels.
 enum { RM_FWD = 0, RM_INV = 1, RM_GAUGE = 2, RM_FULL = 3, RM_KSUM = 4 };
@@ -56,11 +58,10 @@ struct Phys {
   double Dt;      // gauge exponent scale   (GpA/EpA, src/operators.cu:1093-1110)
   double hb2m;    // hbar^2 / (2 mass)      (simple_K, src/operators.cu:467)
   double mass, omega, omegaX, omegaY, omegaZe, gammaY, x0, y0, z0e;
-  double time;    // Dt * i, for time-dependent fields
-  double dxe, dye, dze, xMax, yMax, zMax;   // coordinates for equation fields
   const double *xg, *yg, *zg, *pxg, *pyg, *pzg;
+  const double *vs[3];   // V_SEP: V = vs[0][x] + vs[1][y] + vs[2][z]
+  const double *ks[3];   // K_SEP: K = ks[0][kx] + ks[1][ky] + ks[2][kz]
   const double *Vt, *Kt, *At[3];
-  const EqnProg* veq;  // device copy (V_EQN)
 };
 
 struct SweepArgs {
@@ -73,52 +74,13 @@ struct SweepArgs {
   int gauge_axis;     // axis of the A operator applied by F_GAUGE_POST / gauge modes
   unsigned flags;
   double invN;        // normalisation folded into K (1/N) or A (1/n_axis)
-  const double* norm_in;
-  double* partials;
-  unsigned* counter;
-  double* result;     // norm_out (F_NORM) or kinetic sum (KSUM modes)
-  double dV;
+  const double* norm_in;   // pending renormalisation divisor (F_SCALE_IN)
+  double* partials;        // per-workgroup partial sums (F_NORM, KSUM modes)
 };
 
 // ---------------------------------------------------------------------------
 // Field evaluation (GPUE operators.cu kernels restated per point)
 // ---------------------------------------------------------------------------
-// ---------------------------------------------------------------------------
-// Equation programs: postfix form of GPUE's AST (src/dynamic.cu:527-603).
-// ---------------------------------------------------------------------------
-__device__ __noinline__ static double eval_eqn(const EqnProg* p, double x, double y, double z, double t) {
-  double st[EQ_MAX];
-  int sp = 0;
-  const int n = p->n;
-  for (int i = 0; i < n; ++i) {
-    const int op = p->op[i];
-    if (op == EQ_CONST) { st[sp++] = p->val[i]; continue; }
-    if (op == EQ_X) { st[sp++] = x; continue; }
-    if (op == EQ_Y) { st[sp++] = y; continue; }
-    if (op == EQ_Z) { st[sp++] = z; continue; }
-    if (op == EQ_T) { st[sp++] = t; continue; }
-    const double b = st[--sp], a = st[sp - 1];
-    double r = 0.0;
-    switch (op - EQ_OP) {
-      case 1: r = a + b; break;
-      case 2: r = a - b; break;
-      case 3: r = a * b; break;
-      case 4: r = a / b; break;
-      case 5: r = pow(a, (int)b); break;          // pow_gpu (:88-90)
-      case 6: r = cos(a); break;
-      case 7: r = sin(a); break;
-      case 8: r = tan(a); break;
-      case 9: r = sqrt(a); break;
-      case 10: r = exp(a); break;
-      case 11: { int k = (int)a; r = k == 0 ? j0(b) : k == 1 ? j1(b) : jn(k, b); break; }   // :138-148
-      case 12: { int k = (int)a; r = k == 0 ? y0(b) : k == 1 ? y1(b) : yn(k, b); break; }   // :150-160
-      default: r = 0.0; break;
-    }
-    st[sp - 1] = r;
-  }
-  return sp > 0 ? st[sp - 1] : 0.0;
-}
-
 __device__ __forceinline__ double A_val(const Phys& p, int axis, int xi, int yi, int zi, long gid) {
   if (p.a_kind == A_ROTATION) {           // krotation_Ax/Ay, kconstant_A (:630-647, :621-627)
     if (axis == 0) return -p.yg[yi] * p.omega * p.omegaX;
@@ -129,23 +91,17 @@ __device__ __forceinline__ double A_val(const Phys& p, int axis, int xi, int yi,
   return p.At[axis][gid];
 }
 
+// kharmonic_V (src/operators.cu:994-1008) is a sum of per-axis terms once the
+// rotation gauge is written out (Ax = Ax(y), Ay = Ay(x), Az = 0), so it is
+// carried as three 1-D tables; other fields are full tables.
 __device__ __forceinline__ double V_val(const Phys& p, int xi, int yi, int zi, long gid) {
-  if (p.v_kind == V_HARMONIC) {           // kharmonic_V (:994-1008)
-    double V_x = p.omegaX * (p.xg[xi] + p.x0);
-    double V_y = p.gammaY * p.omegaY * (p.yg[yi] + p.y0);
-    double V_z = 1.0 * p.omegaZe * (p.zg[zi] + p.z0e);
-    double ax = A_val(p, 0, xi, yi, zi, gid), ay = A_val(p, 1, xi, yi, zi, gid),
-           az = A_val(p, 2, xi, yi, zi, gid);
-    return 0.5 * p.mass * ((V_x * V_x + V_y * V_y + V_z * V_z) + (ax * ax + ay * ay + az * az));
-  }
-  if (p.v_kind == V_TABLE) return p.Vt[gid];
-  // time-dependent equation field at find_field's coordinates (dynamic.cu:613-615)
-  return eval_eqn(p.veq, p.dxe * xi - p.xMax, p.dye * yi - p.yMax, p.dze * zi - p.zMax, p.time);
+  if (p.v_kind == V_SEP) return p.vs[0][xi] + p.vs[1][yi] + p.vs[2][zi];
+  return p.Vt[gid];
 }
 
+// simple_K (src/operators.cu:460-469): hbar^2/(2m) (kx^2 + ky^2 + kz^2)
 __device__ __forceinline__ double K_val(const Phys& p, int xi, int yi, int zi, long gid) {
-  if (p.k_kind == K_STANDARD)             // simple_K (:460-469)
-    return p.hb2m * (p.pxg[xi] * p.pxg[xi] + p.pyg[yi] * p.pyg[yi] + p.pzg[zi] * p.pzg[zi]);
+  if (p.k_kind == K_SEP) return p.ks[0][xi] + p.ks[1][yi] + p.ks[2][zi];
   return p.Kt[gid];
 }
 
@@ -197,8 +153,10 @@ __device__ __forceinline__ cx<T> op_A(const Phys& p, cx<T> v, int axis, int xi, 
 }
 
 // ---------------------------------------------------------------------------
-// Deterministic grid reduction of NA doubles: per-workgroup partials, the last
-// workgroup (atomic ticket) sums them in a fixed order.
+// Deterministic two-stage reduction of NA doubles: every workgroup writes its
+// partial sums (plain stores, no fences), a one-workgroup finalize kernel
+// launched after it sums them in a fixed order.  The result is the same for
+// every run and independent of scheduling.
 // ---------------------------------------------------------------------------
 template <int NA>
 __device__ __forceinline__ void block_sum(double (&v)[NA], double* sh /* >= NA * 16 */) {
@@ -226,38 +184,15 @@ __device__ __forceinline__ void block_sum(double (&v)[NA], double* sh /* >= NA *
   }
 }
 
-// Returns true in thread 0 of the last block; `tot` then holds the grid totals.
 template <int NA>
-__device__ __forceinline__ bool grid_sum(double (&v)[NA], double* partials, unsigned* counter, double* sh,
-                                         double (&tot)[NA]) {
-  __shared__ int s_last;
+__device__ __forceinline__ void block_partial(double (&v)[NA], double* partials, double* sh) {
   block_sum<NA>(v, sh);
   const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
   const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int a = 0; a < NA; ++a) partials[(size_t)a * nb + b] = v[a];
-    __threadfence();
-    unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (t == nb - 1);
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  double acc[NA];
-#pragma unroll
-  for (int a = 0; a < NA; ++a) {
-    acc[a] = 0.0;
-    for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
-      acc[a] += __hip_atomic_load(partials + (size_t)a * nb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  block_sum<NA>(acc, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int a = 0; a < NA; ++a) tot[a] = acc[a];
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  return threadIdx.x == 0;
 }
 
 }  // namespace gpue

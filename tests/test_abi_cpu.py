@@ -1,0 +1,61 @@
+"""The C ABI library loads and exports every entry point include/gpue_amd.h
+declares; host-side validation and defaults work without a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import gpue_amd as G
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "gpue_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gpue_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_symbols_exported():
+    lib = G.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    assert sorted(G.ABI_SYMBOLS) == syms
+
+
+def test_params_default_matches_gpue_parser():
+    # src/parser.cu:14-67 and src/init.cu:98-137 (atoms = 1, box_size < 0)
+    p = G.params_default()
+    assert (p.dimnum, p.xDim, p.yDim, p.zDim) == (2, 256, 256, 256)
+    assert (p.gdt, p.dt, p.omegaX, p.omegaY, p.omegaZ, p.gammaY) == (1e-4, 1e-4, 6.283, 6.283, 6.283, 1.0)
+    assert p.hbar == 1.05457148e-34 and p.mass == 1.4431607e-25
+    HBAR, PI, m, a_s = 1.05457148e-34, 3.141592653589793, 1.4431607e-25, 4.76e-9
+    g = 4 * HBAR * HBAR * PI * (a_s / m) * (m * 6.283 / (2 * PI * HBAR)) ** 0.5
+    assert abs(p.g - g) <= 1e-15 * g
+    Rxy = 15 ** 0.2 * (a_s * (m * 6.283 / HBAR) ** 0.5) ** 0.2
+    assert abs(p.xMax - 6 * Rxy * (HBAR / (2 * m * 6.283)) ** 0.5) <= 1e-15 * p.xMax
+
+
+@pytest.mark.parametrize("kw,status", [
+    (dict(dimnum=4), G.GPUE_EINVAL),
+    (dict(precision=16), G.GPUE_EINVAL),
+    (dict(xDim=100, yDim=100), G.GPUE_EUNSUPPORTED),
+    (dict(xDim=2048, yDim=2048), G.GPUE_EUNSUPPORTED),
+    (dict(xMax=-1.0), G.GPUE_EINVAL),
+    (dict(hbar=0.0), G.GPUE_EINVAL),
+])
+def test_create_rejects_before_touching_the_device(kw, status):
+    p = G.make_params(**kw)
+    h = ctypes.c_void_p()
+    assert G.lib().gpue_create(ctypes.byref(p), ctypes.byref(h)) == status
+    assert not h.value
+
+
+def test_null_arguments():
+    lib = G.lib()
+    assert lib.gpue_create(None, None) == G.GPUE_EINVAL
+    assert lib.gpue_energy(None, None, None) == G.GPUE_EINVAL
+    assert lib.gpue_evolve(None, 0, 1, 0, 0, 0.0, None, 0, None) == G.GPUE_EINVAL

@@ -1,0 +1,174 @@
+"""HIP path vs the FP64 oracle and the golden fixtures.
+
+Tolerances (stated here, checked below):
+  FP64:  psi relative L2 <= 1e-10; energy and chemical potential relative <= 1e-10
+  FP32 bulk / FP64 reductions: energy and mu relative <= 1e-6; psi relative L2 <= 1e-5
+"""
+import numpy as np
+import pytest
+
+import gpue_amd as G
+from oracle import oracle as O
+from tests.conftest import rel_l2, seeded_psi
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL32_E = 1e-6
+TOL32_PSI = 1e-5
+
+
+def run_pair(nsteps, gstate, prec=64, phase=True, seed=1, first_step=0, **kw):
+    ctx = G.Context(precision=prec, **kw)
+    orc = O.Oracle(**{k: v for k, v in kw.items() if k != "a_kind"})
+    grids = [ctx.grid(a) for a in range(ctx.dimnum)]
+    psi = seeded_psi(ctx.shape, grids, seed=seed, phase=phase)
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
+    ctx.upload(psi)
+    orc.set_wfc(psi)
+    r, _ = ctx.evolve(nsteps, gstate, first_step=first_step)
+    assert r.steps_done == nsteps
+    for i in range(first_step, first_step + nsteps):
+        orc.step(i, gstate)
+    a, b = ctx.download(), orc.wfc()
+    eg, mg = ctx.energy()
+    eo, mo = orc.energy()
+    ctx.close()
+    return rel_l2(a, b), abs(eg - eo) / abs(eo), abs(mg - mo) / abs(mo)
+
+
+CASES = [
+    ("1d", 1, dict(dimnum=1, xDim=64, xMax=8, g=10, gpe=1)),
+    ("1d-gauge", 1, dict(dimnum=1, xDim=128, xMax=8, g=10, gpe=1, corotating=1, omega=0.3)),
+    ("2d-plain", 2, dict(dimnum=2, xDim=64, yDim=64, xMax=8, yMax=8, g=50, gpe=1)),
+    ("2d-linear", 2, dict(dimnum=2, xDim=32, yDim=32, xMax=6, yMax=6)),
+    ("2d-gauge", 2, dict(dimnum=2, xDim=64, yDim=64, xMax=8, yMax=8, g=50, gpe=1, corotating=1, omega=0.5)),
+    ("2d-aniso", 2, dict(dimnum=2, xDim=128, yDim=128, xMax=9, yMax=9, g=80, gpe=1, corotating=1, omega=0.4,
+                         gammaY=1.15, x0_shift=0.3, y0_shift=-0.2)),
+    ("2d-192", 2, dict(dimnum=2, xDim=192, yDim=192, xMax=10, yMax=10, g=200, gpe=1, corotating=1, omega=0.7)),
+    ("2d-384", 2, dict(dimnum=2, xDim=384, yDim=384, xMax=10, yMax=10, g=200, gpe=1, corotating=1, omega=0.7)),
+    ("2d-1024", 2, dict(dimnum=2, xDim=1024, yDim=1024, xMax=12, yMax=12, g=400, gpe=1, corotating=1, omega=0.7)),
+    ("2d-rect", 2, dict(dimnum=2, xDim=128, yDim=64, xMax=10, yMax=5, g=50, gpe=1)),
+    ("3d-plain", 3, dict(dimnum=3, xDim=32, yDim=32, zDim=32, xMax=6, yMax=6, zMax=6, g=100, gpe=1, omegaZ=2.0)),
+    ("3d-gauge", 3, dict(dimnum=3, xDim=32, yDim=32, zDim=32, xMax=6, yMax=6, zMax=6, g=100, gpe=1, omegaZ=2.0,
+                         corotating=1, omega=0.6)),
+    ("3d-box", 3, dict(dimnum=3, xDim=64, yDim=32, zDim=16, xMax=8, yMax=6, zMax=4, g=60, gpe=1, omegaZ=1.5)),
+]
+
+
+@pytest.mark.parametrize("gstate", [0, 1])
+@pytest.mark.parametrize("name,dim,kw", CASES, ids=[c[0] for c in CASES])
+def test_fp64_parity(name, dim, kw, gstate):
+    steps = 4 if kw.get("xDim", 0) >= 1024 else 12
+    e_psi, e_e, e_mu = run_pair(steps, gstate, first_step=3, gdt=1e-3, dt=5e-4, **kw)
+    assert e_psi <= TOL64
+    assert e_e <= TOL64 and e_mu <= TOL64
+
+
+@pytest.mark.parametrize("name,dim,kw", [c for c in CASES if c[0] in ("2d-gauge", "3d-gauge", "2d-192", "1d")],
+                         ids=["2d-gauge", "3d-gauge", "2d-192", "1d"])
+def test_fp32_bulk_parity(name, dim, kw):
+    e_psi, e_e, e_mu = run_pair(20, 0, prec=32, gdt=1e-3, **kw)
+    assert e_psi <= TOL32_PSI
+    assert e_e <= TOL32_E and e_mu <= TOL32_E
+
+
+def test_matlab_crosscheck_fixture(golden):
+    g = golden("gpe2d_matlab.npz")
+    n = int(g["n"])
+    ctx = G.Context(dimnum=2, xDim=n, yDim=n, xMax=float(g["xmax"]), yMax=float(g["xmax"]), g=float(g["g"]), gpe=1,
+                    gdt=float(g["Dt"]))
+    ctx.upload(g["psi0"])
+    ctx.evolve(1, 0)
+    assert rel_l2(ctx.download(), g["psi_1"]) <= TOL64
+    ctx.evolve(int(g["steps"]) - 1, 0, first_step=1)
+    assert rel_l2(ctx.download(), g["psi_%d" % int(g["steps"])]) <= TOL64
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gauge_order_fixture(golden, dim):
+    g = golden(f"gpue_gauge_{dim}d.npz")
+    n, L = int(g["n"]), float(g["xmax"])
+    ctx = G.Context(dimnum=dim, xDim=n, yDim=n, zDim=n, xMax=L, yMax=L, zMax=L, g=float(g["g"]), gpe=1, corotating=1,
+                    omega=float(g["omega"]), omegaZ=2.0, gdt=float(g["gdt"]), dt=float(g["dt"]))
+    ctx.upload(g["psi0"])
+    ctx.evolve(int(g["steps"]), 0)
+    assert rel_l2(ctx.download(), g["psi_imag"]) <= TOL64
+    ctx.evolve(int(g["steps"]), 1)
+    assert rel_l2(ctx.download(), g["psi_real"]) <= TOL64
+
+
+def test_config1_ground_state_full():
+    # 256^2 over [-10,10)^2, V = (x^2+y^2)/2, g = 500, dt = 1e-3, 2000 imaginary steps,
+    # psi0 = exp(-r^2/2)(1 + 0.01 U[-1,1]) seed 1, normalised
+    kw = dict(dimnum=2, xDim=256, yDim=256, xMax=10, yMax=10, g=500, gpe=1, gdt=1e-3)
+    ctx = G.Context(**kw)
+    orc = O.Oracle(**kw)
+    grids = [ctx.grid(0), ctx.grid(1)]
+    psi = seeded_psi(ctx.shape, grids, seed=1)
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
+    ctx.upload(psi)
+    orc.set_wfc(psi)
+    ctx.evolve(2000, 0)
+    orc.evolve(2000, 0)
+    assert rel_l2(ctx.download(), orc.wfc()) <= TOL64
+    (eg, mg), (eo, mo) = ctx.energy(), orc.energy()
+    assert abs(eg - eo) <= TOL64 * abs(eo) and abs(mg - mo) <= TOL64 * abs(mo)
+
+
+def test_config2_rotating_reduced():
+    # config 2 at full grid (512^2, [-12,12)^2, g = 1000, Omega = 0.9 gauge), shortened to
+    # 100 imaginary steps at 1e-3 then 100 real steps at 1e-4 from a random-phase Gaussian
+    kw = dict(dimnum=2, xDim=512, yDim=512, xMax=12, yMax=12, g=1000, gpe=1, corotating=1, omega=0.9, gdt=1e-3,
+              dt=1e-4)
+    ctx = G.Context(**kw)
+    orc = O.Oracle(**kw)
+    grids = [ctx.grid(0), ctx.grid(1)]
+    psi = seeded_psi(ctx.shape, grids, seed=1, phase=True, noise=0.0)
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
+    ctx.upload(psi)
+    orc.set_wfc(psi)
+    ctx.evolve(100, 0)
+    orc.evolve(100, 0)
+    assert rel_l2(ctx.download(), orc.wfc()) <= TOL64
+    ctx.evolve(100, 1)
+    orc.evolve(100, 1)
+    assert rel_l2(ctx.download(), orc.wfc()) <= TOL64
+    (eg, mg), (eo, mo) = ctx.energy(), orc.energy()
+    assert abs(eg - eo) <= TOL64 * abs(eo) and abs(mg - mo) <= TOL64 * abs(mo)
+
+
+@pytest.mark.parametrize("prec,tol", [(64, TOL64), (32, TOL32_E)])
+def test_config3_reduced(prec, tol):
+    # config 3 physics (V = (x^2+y^2+4z^2)/2, g = 5000, Omega = 0.6 gauge, dt = 5e-4)
+    # on 64^3 for 30 imaginary steps
+    kw = dict(dimnum=3, xDim=64, yDim=64, zDim=64, xMax=8, yMax=8, zMax=8, g=5000, gpe=1, omegaZ=2.0, corotating=1,
+              omega=0.6, gdt=5e-4)
+    ctx = G.Context(precision=prec, **kw)
+    orc = O.Oracle(**kw)
+    grids = [ctx.grid(a) for a in range(3)]
+    psi = seeded_psi(ctx.shape, grids, seed=1)
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
+    ctx.upload(psi)
+    orc.set_wfc(psi)
+    ctx.evolve(30, 0)
+    orc.evolve(30, 0)
+    (eg, mg), (eo, mo) = ctx.energy(), orc.energy()
+    assert abs(eg - eo) <= tol * abs(eo) and abs(mg - mo) <= tol * abs(mo)
+    assert rel_l2(ctx.download(), orc.wfc()) <= (TOL64 if prec == 64 else TOL32_PSI)
+
+
+def test_energy_cadence_and_early_stop():
+    kw = dict(dimnum=2, xDim=64, yDim=64, xMax=6, yMax=6, g=20, gpe=1, gdt=1e-2)
+    ctx = G.Context(**kw)
+    orc = O.Oracle(**kw)
+    grids = [ctx.grid(0), ctx.grid(1)]
+    psi = seeded_psi(ctx.shape, grids, seed=9)
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
+    ctx.upload(psi)
+    orc.set_wfc(psi)
+    r, eg = ctx.evolve(3000, 0, energy_every=10, thr=1e-7)
+    done, eo = orc.evolve(3000, 0, energy_every=10, thr=1e-7)
+    assert r.early_stop == 1 and r.steps_done == done
+    assert len(eg) == len(eo)
+    assert np.max(np.abs(eg - eo) / np.abs(eo)) <= TOL64

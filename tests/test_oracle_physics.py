@@ -1,0 +1,163 @@
+"""The oracle's step, parSum and energy pinned by the reference's KATs and by
+closed forms (src/unit_test.cu:841-947, 1162-1278)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import rel_l2, seeded_psi
+
+
+def test_parsum_kat_2d_3d():
+    # src/unit_test.cu:841-947: 2+2i everywhere, dx = dy = dz = 0.1
+    w = np.full((64, 64), 2 + 2j)
+    O.parsum_raw(w, 2, 0.1, 0.1, 0.1)
+    assert np.all(w.real == 2 / np.sqrt(32768.0 * 0.1 * 0.1))
+    assert np.all(w.imag == 2 / np.sqrt(32768.0 * 0.1 * 0.1))
+    w = np.full((16, 16, 16), 2 + 2j)
+    O.parsum_raw(w, 3, 0.1, 0.1, 0.1)
+    assert np.all(w.real == 2 / np.sqrt(32768.0 * 0.1 * 0.1 * 0.1))
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_ho_ground_state_energy(dim):
+    # evolve_test (src/unit_test.cu:1162-1278): E = d * hbar omega / 2 within 1 %;
+    # here in oscillator units, and tighter: the linear ground state is a Gaussian.
+    n = 32 if dim == 3 else 64
+    o = O.Oracle(dimnum=dim, xDim=n, yDim=n, zDim=n, xMax=7, yMax=7, zMax=7, gdt=1e-3, dt=1e-3, omegaZ=1.0)
+    grids = [o.grid(a) for a in range(dim)]
+    o.set_wfc(seeded_psi(o.shape, grids, seed=dim))
+    o.parsum()
+    o.evolve(4000 if dim < 3 else 2500, 0)
+    e, mu = o.energy()
+    assert abs(e - dim / 2) < 1e-5 * dim
+    assert abs(mu - dim / 2) < 1e-5 * dim
+    # real time: the stationary state keeps its energy and norm
+    o.evolve(2000, 1)
+    e2, _ = o.energy()
+    psi = o.wfc()
+    assert abs(e2 - e) < 1e-9
+    assert abs(np.sum(np.abs(psi) ** 2) * o.dV - 1) < 1e-12
+
+
+def test_plane_wave_kinetic_energy():
+    # K = hbar^2 k^2 / 2m for a grid plane wave (energy_calc kinetic term, src/split_op.cu:256-273)
+    n, L = 64, 8.0
+    o = O.Oracle(dimnum=2, xDim=n, yDim=n, xMax=L, yMax=L, hbar=1.3, mass=0.7)
+    x = o.grid(0)
+    kx = 5 * np.pi / L
+    ky = -3 * np.pi / L
+    psi = np.exp(1j * (kx * x[None, :] + ky * x[:, None]))
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * o.dV)
+    o.set_wfc(psi)
+    V = o.field(O.FIELD_V)
+    e, _ = o.energy()
+    expect = 1.3 ** 2 * (kx ** 2 + ky ** 2) / (2 * 0.7) + np.sum(V) * o.dV / (4 * L * L) * 1.0
+    assert abs(e - expect) < 1e-10 * abs(expect)
+
+
+def test_gauge_energy_forward_difference():
+    # gauge term of energy_calc: Re conj(psi) i hbar (Ax Dx psi + Ay Dy psi), D = forward
+    # difference on the flat array (src/split_op.cu:312-360, derive src/kernels.cu:58-71).
+    # For psi = exp(i k x) the interior value is -hbar Ax sin(k dx)/dx |psi|^2.
+    n, L, om = 32, 6.0, 0.4
+    o = O.Oracle(dimnum=2, xDim=n, yDim=n, xMax=L, yMax=L, corotating=1, omega=om)
+    o2 = O.Oracle(dimnum=2, xDim=n, yDim=n, xMax=L, yMax=L, corotating=0, omega=om)
+    x = o.grid(0)
+    dx = x[1] - x[0]
+    k = 4 * np.pi / (2 * L)
+    psi = np.exp(1j * k * x)[None, :] * np.ones((n, 1))
+    o.set_wfc(psi)
+    o2.set_wfc(psi)
+    Ax = o.field(O.FIELD_AX)
+    Ay = o.field(O.FIELD_AY)
+    # closed form per point: hbar * Re[conj(psi) * i * (Ax Dx psi + Ay Dy psi)]
+    dxpsi = (np.exp(1j * k * dx) - 1) / dx * psi
+    dxpsi[:, -1] = (np.roll(psi.reshape(-1), -1).reshape(n, n)[:, -1] - psi[:, -1]) / dx
+    dxpsi[-1, -1] = psi[-1, -1] / dx
+    dypsi = np.zeros_like(psi)
+    dypsi[-1, :] = psi[-1, :] / dx
+    term = np.real(np.conj(psi) * 1j * (Ax * dxpsi + Ay * dypsi))
+    e_g, _ = o.energy()
+    e_0, _ = o2.energy()
+    assert abs((e_g - e_0) - np.sum(term) * o.dV) < 1e-10 * max(1.0, abs(e_g))
+
+
+def test_thomas_fermi_bound():
+    # 2D TF limit for V = r^2/2: mu_TF = sqrt(g/pi), E_TF = 2/3 mu_TF (sanity bound)
+    g = 2000.0
+    o = O.Oracle(dimnum=2, xDim=128, yDim=128, xMax=12, yMax=12, g=g, gpe=1, gdt=2e-3)
+    grids = [o.grid(0), o.grid(1)]
+    o.set_wfc(seeded_psi(o.shape, grids, seed=2))
+    o.parsum()
+    o.evolve(3000, 0)
+    e, mu = o.energy()
+    mu_tf = np.sqrt(g / np.pi)
+    assert 0.97 * mu_tf < mu < 1.06 * mu_tf
+    assert 0.97 * (2 / 3) * mu_tf < e < 1.06 * (2 / 3) * mu_tf
+
+
+def test_matlab_gpe2d_crosscheck(golden):
+    # matlab/GPE_2d.m imaginary-time loop (lines 87-110) transcribed independently in numpy
+    g = golden("gpe2d_matlab.npz")
+    n = int(g["n"])
+    o = O.Oracle(dimnum=2, xDim=n, yDim=n, xMax=float(g["xmax"]), yMax=float(g["xmax"]), g=float(g["g"]), gpe=1,
+                 gdt=float(g["Dt"]))
+    o.set_wfc(g["psi0"])
+    o.step(0, 0)
+    assert rel_l2(o.wfc(), g["psi_1"]) < 1e-13
+    for i in range(1, int(g["steps"])):
+        o.step(i, 0)
+    assert rel_l2(o.wfc(), g["psi_%d" % int(g["steps"])]) < 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gauge_order_crosscheck(golden, dim):
+    # GPUE's step order with alternating per-axis gauge operators, transcribed in numpy
+    g = golden(f"gpue_gauge_{dim}d.npz")
+    n = int(g["n"])
+    L = float(g["xmax"])
+    o = O.Oracle(dimnum=dim, xDim=n, yDim=n, zDim=n, xMax=L, yMax=L, zMax=L, g=float(g["g"]), gpe=1,
+                 corotating=1, omega=float(g["omega"]), omegaZ=2.0, gdt=float(g["gdt"]), dt=float(g["dt"]))
+    o.set_wfc(g["psi0"])
+    steps = int(g["steps"])
+    for i in range(steps):
+        o.step(i, 0)
+    assert rel_l2(o.wfc(), g["psi_imag"]) < 1e-12
+    for i in range(steps):
+        o.step(i, 1)
+    assert rel_l2(o.wfc(), g["psi_real"]) < 1e-12
+
+
+def test_real_time_unitarity_and_energy():
+    o = O.Oracle(dimnum=2, xDim=64, yDim=64, xMax=8, yMax=8, g=80, gpe=1, corotating=1, omega=0.5, dt=5e-4)
+    grids = [o.grid(0), o.grid(1)]
+    o.set_wfc(seeded_psi(o.shape, grids, seed=5, phase=False))
+    o.parsum()
+    e0, _ = o.energy()
+    o.evolve(400, 1)
+    psi = o.wfc()
+    assert abs(np.sum(np.abs(psi) ** 2) * o.dV - 1) < 1e-12
+    e1, _ = o.energy()
+    assert abs(e1 - e0) < 2e-2 * abs(e0)
+
+
+def test_vortex_fixtures(golden):
+    d = golden("vortex_cases.npz")
+    for c in ["v1", "v2", "v3"]:
+        m, found = O.find_vortex(d[c + "_psi"], float(d[c + "_radius"]), d["x"])
+        assert np.array_equal(m, d[c + "_marker"]), c
+        assert found == int(np.count_nonzero(d[c + "_marker"])), c
+    lst = O.vort_pos(d["v1_marker"])
+    assert [tuple(r) for r in lst] == [(20, 20, 1), (20, 40, 1), (40, 30, 1), (45, 45, 1)]
+
+
+def test_early_stop_rule():
+    # src/evolution.cu:899-902: stop when |E_old - E| < thr * E_old in imaginary time
+    o = O.Oracle(dimnum=2, xDim=32, yDim=32, xMax=6, yMax=6, gdt=1e-2)
+    grids = [o.grid(0), o.grid(1)]
+    o.set_wfc(seeded_psi(o.shape, grids, seed=9))
+    o.parsum()
+    done, en = o.evolve(5000, 0, energy_every=10, thr=1e-6)
+    assert done < 5000 and done % 10 == 1
+    assert abs(en[-1] - en[-2]) < 1e-6 * en[-2]
+    assert abs(en[-2] - en[-3]) >= 1e-6 * en[-3]
