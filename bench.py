@@ -110,6 +110,115 @@ def cpu_baseline(budget_s):
                       f"{steps} imaginary-time steps of the job, {el:.1f} s"}
 
 
+def batched_sweep(G, n_inst=64, seed=4, max_steps=3000, tol=1e-8):
+    """64 independent 2D imaginary-time solves on uneven grids, FP32 bulk with
+    FP64 norm/energy, advanced together through one batch (one launch per sweep
+    per shape, two steps per graph replay) until each instance's energy changes
+    by less than tol (relative) over 100 steps."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([128, 192, 256, 384, 512], n_inst)
+    ctxs = []
+    tuned = {}
+    for k in range(n_inst):
+        n = int(sizes[k])
+        L = 10.0 if n <= 256 else 12.0
+        cfg = dict(dimnum=2, xDim=n, yDim=n, xMax=L, yMax=L, g=float(rng.uniform(100, 2000)), gpe=1,
+                   omega=float(rng.uniform(0, 0.95)), corotating=1, omegaX=1.0,
+                   omegaY=float(rng.uniform(0.8, 1.2)), gdt=1e-3, dt=1e-4)
+        ctx, _ = make_ctx(G, 32, cfg, seed=100 + k, phase=True, noise=0.01)
+        if n not in tuned:
+            tuned[n] = ctx.autotune()
+        else:
+            ctx.set_variants(tuned[n])
+        ctxs.append(ctx)
+    B = G.Batch(ctxs)
+    active = [True] * n_inst
+    e_prev = [None] * n_inst
+    steps = [0] * n_inst
+    point_steps = 0.0
+    bytes_total = 0.0
+    per_step_bytes = [c.step_bytes(0)[0] for c in ctxs]
+    t0 = time.perf_counter()
+    done_at = [None] * n_inst
+    while any(active) and max(steps) < max_steps:
+        B.evolve(100, 0)
+        for k, c in enumerate(ctxs):
+            if not active[k]:
+                continue
+            steps[k] += 100
+            point_steps += 100 * c.N
+            bytes_total += 100 * per_step_bytes[k]
+            e, _ = c.energy()
+            if e_prev[k] is not None and abs(e - e_prev[k]) < tol * abs(e_prev[k]):
+                active[k] = False
+                done_at[k] = time.perf_counter() - t0
+                B.set_active(k, False)
+            e_prev[k] = e
+    el = time.perf_counter() - t0
+    B.close()
+    for c in ctxs:
+        c.close()
+    conv = sum(1 for d in done_at if d is not None)
+    return {"instances": n_inst, "sizes": {int(n): int(np.sum(sizes == n)) for n in sorted(set(sizes))},
+            "converged": conv, "max_steps": max_steps, "wall_s": round(el, 3),
+            "instance_steps_per_s": round(sum(steps) / el, 1),
+            "grid_point_steps_per_s": round(point_steps / el, 1),
+            "effective_GBps": round(bytes_total / el / 1e9, 1),
+            "variants_by_size": {int(k): list(v) for k, v in tuned.items()},
+            "steps_to_converge": {"min": int(min(steps)), "max": int(max(steps))}}
+
+
+def serving_stream(G, n_req=48, rate=200.0, seed=5, budget=250e9):
+    """Poisson arrivals (mean rate req/s) of 2D ground states (128^2-1024^2) and
+    3D quench runs (64^3-256^3: imaginary-time preparation, g quenched by
+    U[0.5,2], 1000 real-time steps), each in FP64 or FP32-bulk, admitted by the
+    HBM budget; reports throughput and p50/p99 latency (submit to result)."""
+    rng = np.random.default_rng(seed)
+    sv = G.Server(budget_bytes=budget, max_slots=32)
+    reqs = []
+    for k in range(n_req):
+        prec = 64 if rng.uniform() < 0.5 else 32
+        if rng.uniform() < 0.6:
+            n = int(rng.choice([128, 192, 256, 384, 512, 1024]))
+            L = 10.0 if n <= 256 else 12.0
+            p = G.make_params(dimnum=2, xDim=n, yDim=n, xMax=L, yMax=L, g=float(rng.uniform(100, 1000)), gpe=1,
+                              gdt=1e-3, dt=1e-4, precision=prec)
+            kw = dict(max_imag_steps=2000, energy_tol=1e-10 if prec == 64 else 1e-6, check_every=100)
+            kind = "2d_ground"
+        else:
+            n = int(rng.choice([64, 128, 192, 256]))
+            p = G.make_params(dimnum=3, xDim=n, yDim=n, zDim=n, xMax=8.0, yMax=8.0, zMax=8.0,
+                              g=float(rng.uniform(100, 2000)), gpe=1, omegaZ=2.0, gdt=5e-4, dt=1e-4,
+                              precision=prec)
+            kw = dict(max_imag_steps=100, energy_tol=0.0, real_steps=1000,
+                      quench_factor=float(rng.uniform(0.5, 2.0)), check_every=100)
+            kind = "3d_quench"
+        reqs.append((k, p, kw, kind, n, prec))
+    gaps = rng.exponential(1.0 / rate, n_req)
+    t0 = time.perf_counter()
+    t_next = t0
+    for (k, p, kw, kind, n, prec), gap in zip(reqs, gaps):
+        t_next += gap
+        while time.perf_counter() < t_next:
+            time.sleep(min(1e-3, max(0.0, t_next - time.perf_counter())))
+        sv.submit(k, p, seed=1000 + k, **kw)
+    sv.drain(900.0)
+    res = sv.poll(4096)
+    el = time.perf_counter() - t0
+    sv.close()
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res]) * 1e3
+    bad = [r for r in res if r["status"] != 0]
+    kinds = {}
+    for (k, p, kw, kind, n, prec) in reqs:
+        kinds[kind] = kinds.get(kind, 0) + 1
+    return {"requests": n_req, "mean_rate_req_s": rate, "failed": len(bad), "mix": kinds,
+            "wall_s": round(el, 3), "throughput_req_s": round(len(res) / el, 2),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                           "p99": round(float(np.percentile(lat, 99)), 2),
+                           "max": round(float(lat.max()), 2)},
+            "budget_GB": budget / 1e9, "max_slots": 32}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,7 +227,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the 256^3 / latency extras")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=["512", "256c"], default=None, help="profiling: run one workload only")
+    ap.add_argument("--only", choices=["512", "256c", "batch", "serve"], default=None,
+                    help="profiling: run one workload only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; every solve runs on a single MI355X "
@@ -221,6 +331,14 @@ def main():
         extra["solve_latency_256sq_2000steps_ms"] = {"p50": round(float(np.percentile(lat, 50)), 3),
                                                      "p99": round(float(np.percentile(lat, 99)), 3),
                                                      "samples": len(lat)}
+
+    # ---------------- batched sweep and serving stream ----------------
+    if not args.no_extra and args.only in (None, "batch"):
+        extra["batched_sweep_64_uneven_fp32"] = batched_sweep(G)
+        log("batched sweep:", json.dumps(extra["batched_sweep_64_uneven_fp32"]))
+    if not args.no_extra and args.only in (None, "serve"):
+        extra["serving_stream"] = serving_stream(G)
+        log("serving stream:", json.dumps(extra["serving_stream"]))
 
     cpu = None
     if not args.no_cpu and args.only is None:

@@ -34,7 +34,10 @@ ABI_SYMBOLS = [
     "gpue_params_default", "gpue_create", "gpue_destroy", "gpue_last_error", "gpue_set_field",
     "gpue_set_field_eqn", "gpue_upload_wfc", "gpue_download_wfc", "gpue_evolve", "gpue_parsum",
     "gpue_energy", "gpue_vortex_marker_2d", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
-    "gpue_sync", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_eqn_parse", "gpue_eqn_eval",
+    "gpue_sync", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_eqn_parse", "gpue_eqn_eval",
+    "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_evolve", "gpue_batch_last_error", "gpue_batch_destroy",
+    "gpue_server_create", "gpue_server_submit", "gpue_server_poll", "gpue_server_drain", "gpue_server_clock",
+    "gpue_server_destroy",
 ]
 
 
@@ -100,6 +103,9 @@ def lib():
         L.gpue_get_grid.argtypes = [vp, ctypes.c_int, dp]
         L.gpue_set_g.argtypes = [vp, ctypes.c_double]
         L.gpue_sync.argtypes = [vp]
+        L.gpue_autotune.argtypes = [vp, ctypes.c_int, ip]
+        L.gpue_set_variants.argtypes = [vp, ip]
+        L.gpue_set_graph_steps.argtypes = [vp, ctypes.c_int]
         L.gpue_set_timing.argtypes = [vp, ctypes.c_int]
         L.gpue_timing_get.argtypes = [vp, ctypes.c_int, ctypes.POINTER(cp), ctypes.POINTER(ctypes.c_long), dp]
         L.gpue_step_bytes.argtypes = [vp, ctypes.c_int, dp, dp]
@@ -108,6 +114,21 @@ def lib():
         L.gpue_eqn_eval.argtypes = [ctypes.POINTER(EqnNode), ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double]
         L.gpue_eqn_eval.restype = ctypes.c_double
+        L.gpue_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+        L.gpue_batch_set_active.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.gpue_batch_evolve.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.gpue_batch_last_error.argtypes = [vp]
+        L.gpue_batch_last_error.restype = cp
+        L.gpue_batch_destroy.argtypes = [vp]
+        L.gpue_batch_destroy.restype = None
+        L.gpue_server_create.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(vp)]
+        L.gpue_server_submit.argtypes = [vp, ctypes.c_void_p, ctypes.c_ulonglong]
+        L.gpue_server_poll.argtypes = [vp, ctypes.c_void_p, ctypes.c_int]
+        L.gpue_server_drain.argtypes = [vp, ctypes.c_double]
+        L.gpue_server_clock.argtypes = [vp]
+        L.gpue_server_clock.restype = ctypes.c_double
+        L.gpue_server_destroy.argtypes = [vp]
+        L.gpue_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -271,6 +292,18 @@ class Context:
     def sync(self):
         self._check(lib().gpue_sync(self._h), "sync")
 
+    def autotune(self, reps=5):
+        v = np.zeros(3, dtype=np.int32)
+        self._check(lib().gpue_autotune(self._h, reps, _ip(v)), "autotune")
+        return tuple(int(a) for a in v)
+
+    def set_graph_steps(self, steps):
+        self._check(lib().gpue_set_graph_steps(self._h, steps), "set_graph_steps")
+
+    def set_variants(self, variants):
+        v = np.ascontiguousarray(variants, dtype=np.int32)
+        self._check(lib().gpue_set_variants(self._h, _ip(v)), "set_variants")
+
     # measurement --------------------------------------------------------------
     def set_timing(self, on):
         self._check(lib().gpue_set_timing(self._h, 1 if on else 0), "set_timing")
@@ -293,3 +326,117 @@ class Context:
         s = ctypes.c_double()
         self._check(lib().gpue_step_bytes(self._h, gstate, ctypes.byref(b), ctypes.byref(s)), "step_bytes")
         return b.value, s.value
+
+
+class Batch:
+    """Many independent contexts advanced together (gpue_batch): one launch per
+    sweep covers every instance of a shape; two steps are replayed from a graph."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c._h.value for c in self.contexts])
+        h = ctypes.c_void_p()
+        st = lib().gpue_batch_create(arr, len(self.contexts), ctypes.byref(h))
+        if st != GPUE_OK:
+            raise GpueError(st, "gpue_batch_create failed")
+        self._h = h
+
+    def _check(self, st, what):
+        if st != GPUE_OK:
+            msg = lib().gpue_batch_last_error(self._h)
+            raise GpueError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_active(self, index, active):
+        self._check(lib().gpue_batch_set_active(self._h, index, 1 if active else 0), "batch_set_active")
+
+    def evolve(self, nsteps, gstate):
+        self._check(lib().gpue_batch_evolve(self._h, nsteps, gstate), "batch_evolve")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gpue_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("params", Params), ("seed", ctypes.c_ulonglong), ("random_phase", ctypes.c_int),
+        ("check_every", ctypes.c_int), ("max_imag_steps", ctypes.c_int), ("energy_tol", ctypes.c_double),
+        ("real_steps", ctypes.c_int), ("quench_factor", ctypes.c_double),
+        ("psi_out", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_ulonglong), ("status", ctypes.c_int), ("imag_steps", ctypes.c_int),
+        ("real_steps", ctypes.c_int), ("energy", ctypes.c_double), ("mu", ctypes.c_double),
+        ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+    ]
+
+
+class Server:
+    """Continuous-batching solve service (gpue_server): requests are admitted in
+    arrival order while a slot and the HBM budget allow; each admitted request
+    runs on its own stream, overlapping with the others."""
+
+    def __init__(self, budget_bytes, max_slots, device=0):
+        h = ctypes.c_void_p()
+        st = lib().gpue_server_create(device, float(budget_bytes), int(max_slots), ctypes.byref(h))
+        if st != GPUE_OK:
+            raise GpueError(st, "gpue_server_create failed")
+        self._h = h
+        self._keep = {}   # id -> psi_out array, alive until its result is polled
+
+    def submit(self, rid, params, seed=1, random_phase=False, check_every=100, max_imag_steps=1000,
+               energy_tol=1e-8, real_steps=0, quench_factor=1.0, psi_out=None):
+        r = Request()
+        r.params = params
+        r.seed, r.random_phase = int(seed), 1 if random_phase else 0
+        r.check_every, r.max_imag_steps, r.energy_tol = int(check_every), int(max_imag_steps), float(energy_tol)
+        r.real_steps, r.quench_factor = int(real_steps), float(quench_factor)
+        if psi_out is not None:
+            assert psi_out.dtype == np.complex128 and psi_out.flags.c_contiguous
+            r.psi_out = psi_out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            self._keep[int(rid)] = psi_out
+        st = lib().gpue_server_submit(self._h, ctypes.byref(r), int(rid))
+        if st != GPUE_OK:
+            self._keep.pop(int(rid), None)
+            raise GpueError(st, "gpue_server_submit failed")
+
+    def poll(self, max_results=1024):
+        buf = (Result * max_results)()
+        n = lib().gpue_server_poll(self._h, buf, max_results)
+        out = []
+        for k in range(n):
+            r = buf[k]
+            self._keep.pop(int(r.id), None)
+            out.append(dict(id=int(r.id), status=int(r.status), imag_steps=int(r.imag_steps),
+                            real_steps=int(r.real_steps), energy=float(r.energy), mu=float(r.mu),
+                            t_submit=float(r.t_submit), t_start=float(r.t_start), t_done=float(r.t_done)))
+        return out
+
+    def drain(self, timeout_s=600.0):
+        st = lib().gpue_server_drain(self._h, float(timeout_s))
+        if st != GPUE_OK:
+            raise GpueError(st, "gpue_server_drain timed out")
+
+    def clock(self):
+        return float(lib().gpue_server_clock(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gpue_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

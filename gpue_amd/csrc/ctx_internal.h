@@ -1,0 +1,91 @@
+// ctx_internal.h -- the context behind the opaque gpue_ctx handle, shared by
+// the single-problem ABI (gpue_ctx.hip) and the batch runtime (gpue_batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "gpue_amd.h"
+#include "misc_kernels.h"
+
+namespace gpue {
+struct Sweep {
+  bool row;
+  int axis;
+  int mode;
+  unsigned flags;
+  double invN;
+  int var = -1;   // kernel variant; -1: the context's choice for this axis
+};
+
+constexpr int TIMING_PAIRS = 256;
+
+}  // namespace gpue
+
+struct gpue_ctx {
+  gpue_params p;
+  int prec = 64;
+  size_t esz = 16;   // bytes per complex element
+  long N = 0;
+  int dims[3] = {1, 1, 1};
+  double d[3] = {1, 1, 1};
+  std::vector<double> grid[6];
+  hipStream_t stream = nullptr;
+  void* psi = nullptr;
+  void* tmp = nullptr;
+  double* dgrid[6] = {};
+  std::map<int, void*> tw;   // N -> twiddle table in the working precision
+  int var[3] = {0, 0, 0};      // kernel variant per axis (row sweeps: axis 0)
+  bool tuned = false;
+  // steady-state step graphs: graph_steps consecutive steps (even count, starting
+  // at an even step, norm pending) captured once per gstate and replayed
+  int graph_steps = 8;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  double* partials = nullptr;
+  size_t partials_n = 0;
+  double* results = nullptr;   // [0] norm  [1] kinetic  [2..5] position terms
+  int v_kind = gpue::V_SEP, a_kind = gpue::A_ROTATION, k_kind = gpue::K_SEP;
+  bool v_user = false;         // V supplied by the caller (table or equation)
+  double* vs[3] = {};          // separable V / K value tables
+  double* ks[3] = {};
+  double* kf[2][3] = {};       // per-axis K operator factors for gstate 0 / 1
+  double* Vt = nullptr;
+  double* Kt = nullptr;
+  double* At[3] = {};
+  bool az_zero = true;
+  bool v_time = false;         // V is a time-dependent equation, refreshed each step
+  gpue::EqnProg veq;
+  bool pending_norm = false;   // results[0] holds a norm not yet applied (else it holds 1.0)
+  int next_step = 0;           // step index after the last evolve (batch parity)
+  double last_energy = 0.0;
+  std::string err;
+  // vortex scratch
+  unsigned char* vmask = nullptr;
+  double vmask_radius = -1.0;
+  int* vcand = nullptr;
+  int* vmarker = nullptr;
+  int* vfound = nullptr;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_kind;
+  int ev_used = 0;
+  std::vector<std::string> kind_names;
+  std::vector<long> kind_n;
+  std::vector<double> kind_ms;
+};
+
+
+namespace gpue {
+gpue_status fail(gpue_ctx* c, gpue_status s, const std::string& msg);
+Phys make_phys(const gpue_ctx* c, int gstate, double time);
+void schedule(const gpue_ctx* c, int gstate, int i, bool pending, std::vector<Sweep>& out);
+gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void* in, void* out);
+gpue_status apply_pending(gpue_ctx* c);
+gpue_status evolve_async(gpue_ctx* c, int first_step, int nsteps, int gstate);
+gpue_status energy_launch(gpue_ctx* c, double* host_dst);
+void energy_combine(const gpue_ctx* c, const double* terms, double* energy, double* mu);
+}  // namespace gpue

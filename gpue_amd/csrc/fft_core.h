@@ -153,31 +153,48 @@ struct DFT<T, 16, SIGN> {
 
 // ---------------------------------------------------------------------------
 // Plans: E points per thread and the radix sequence for each supported N.
+// Variant 0 ("wide") holds 12-24 points per thread and needs the fewest LDS
+// exchanges; variant 1 ("narrow") holds 4-12 points per thread, so a line is
+// spread over more lanes -- more parallelism for small, latency-bound grids.
+// The autotuner picks one per sweep (gpue_ctx.hip).
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int V>
 struct Plan;
-#define GPUE_PLAN(N_, E_, NP_, ...)                                     \
+#define GPUE_PLAN(N_, V_, E_, NP_, ...)                                 \
   template <>                                                           \
-  struct Plan<N_> {                                                     \
+  struct Plan<N_, V_> {                                                 \
     static constexpr int N = N_, E = E_, TPL = N_ / E_, NP = NP_;       \
     static constexpr int R[NP_] = {__VA_ARGS__};                        \
   };
-GPUE_PLAN(16, 16, 1, 16)
-GPUE_PLAN(32, 16, 2, 16, 2)
-GPUE_PLAN(64, 16, 2, 16, 4)
-GPUE_PLAN(128, 16, 2, 16, 8)
-GPUE_PLAN(192, 12, 3, 12, 4, 4)
-GPUE_PLAN(256, 16, 2, 16, 16)
-GPUE_PLAN(384, 24, 3, 12, 8, 4)
-GPUE_PLAN(512, 16, 3, 16, 16, 2)
-GPUE_PLAN(1024, 16, 3, 16, 16, 4)
+GPUE_PLAN(16, 0, 16, 1, 16)
+GPUE_PLAN(16, 1, 4, 2, 4, 4)
+GPUE_PLAN(32, 0, 16, 2, 16, 2)
+GPUE_PLAN(32, 1, 8, 2, 8, 4)
+GPUE_PLAN(48, 0, 12, 2, 12, 4)
+GPUE_PLAN(48, 1, 12, 3, 3, 4, 4)
+GPUE_PLAN(64, 0, 16, 2, 16, 4)
+GPUE_PLAN(64, 1, 8, 2, 8, 8)
+GPUE_PLAN(96, 0, 24, 2, 12, 8)
+GPUE_PLAN(96, 1, 12, 3, 12, 4, 2)
+GPUE_PLAN(128, 0, 16, 2, 16, 8)
+GPUE_PLAN(128, 1, 8, 3, 8, 8, 2)
+GPUE_PLAN(192, 0, 12, 3, 12, 4, 4)
+GPUE_PLAN(192, 1, 12, 4, 3, 4, 4, 4)
+GPUE_PLAN(256, 0, 16, 2, 16, 16)
+GPUE_PLAN(256, 1, 8, 3, 8, 8, 4)
+GPUE_PLAN(384, 0, 24, 3, 12, 8, 4)
+GPUE_PLAN(384, 1, 12, 4, 12, 4, 4, 2)
+GPUE_PLAN(512, 0, 16, 3, 16, 16, 2)
+GPUE_PLAN(512, 1, 8, 3, 8, 8, 8)
+GPUE_PLAN(1024, 0, 16, 3, 16, 16, 4)
+GPUE_PLAN(1024, 1, 8, 4, 8, 8, 8, 2)
 #undef GPUE_PLAN
 
 enum { ORD_FWD = 0, ORD_REV = 1 };
 
-template <int N, int ORD>
+template <int N, int V, int ORD>
 struct Seq {
-  using P = Plan<N>;
+  using P = Plan<N, V>;
   static constexpr int radix(int p) { return ORD == ORD_FWD ? P::R[p] : P::R[P::NP - 1 - p]; }
   static constexpr int first() { return radix(0); }
   static constexpr int last() { return radix(P::NP - 1); }
@@ -190,9 +207,8 @@ struct Seq {
 };
 
 // Point index held in register slot r of thread tid in the boundary layout of radix R.
-template <int N, int R>
+template <int N, int TPL, int R>
 __device__ __forceinline__ constexpr int bpos(int tid, int r) {
-  constexpr int TPL = Plan<N>::TPL;
   return tid + TPL * (r / R) + (r % R) * (N / R);
 }
 
@@ -205,9 +221,9 @@ __device__ __forceinline__ cx<T> twl(const cx<T>* __restrict__ tw, int m) {
 }
 
 // Compute pass p (radix R, L) on registers v[E] of thread tid.
-template <typename T, int N, int R, int L, int SIGN>
+template <typename T, int N, int E, int R, int L, int SIGN>
 __device__ __forceinline__ void pass_compute(cx<T>* v, int tid, const cx<T>* __restrict__ tw) {
-  constexpr int E = Plan<N>::E, TPL = Plan<N>::TPL, NB = E / R;
+  constexpr int TPL = N / E, NB = E / R;
   constexpr int STEP = N / (L * R);
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
@@ -223,9 +239,8 @@ __device__ __forceinline__ void pass_compute(cx<T>* v, int tid, const cx<T>* __r
 }
 
 // Output position of register slot (q, s) after pass (R, L).
-template <int N, int R, int L>
+template <int N, int TPL, int R, int L>
 __device__ __forceinline__ int pass_out(int tid, int q, int s) {
-  constexpr int TPL = Plan<N>::TPL;
   const int j = tid + TPL * q;
   const int k = j % L;
   return (j / L) * L * R + k + s * L;
@@ -246,44 +261,34 @@ struct LdsCol {
 // Full transform on registers: input in boundary layout of first radix,
 // output in boundary layout of last radix.  Inner passes exchange via LDS.
 // `active` predicates LDS traffic for padding threads (they still hit barriers).
-template <typename T, int N, int ORD, int SIGN, typename LA>
-__device__ __forceinline__ void fft_regs(cx<T>* v, int tid, const cx<T>* __restrict__ tw, cx<T>* lds,
-                                         const LA& la, bool active) {
-  using S = Seq<N, ORD>;
-  constexpr int NP = Plan<N>::NP;
-  constexpr int E = Plan<N>::E;
-  // pass 0
-  pass_compute<T, N, S::radix(0), 1, SIGN>(v, tid, tw);
-  if constexpr (NP > 1) {
-    // exchange 0 -> 1
-    __syncthreads();  // previous users of LDS are done
-    if (active) {
+template <typename T, int N, int V, int ORD, int SIGN, int P, typename LA>
+__device__ __forceinline__ void fft_from_pass(cx<T>* v, int tid, const cx<T>* __restrict__ tw, cx<T>* lds,
+                                              const LA& la, bool active) {
+  using S = Seq<N, V, ORD>;
+  constexpr int NP = Plan<N, V>::NP, E = Plan<N, V>::E, TPL = Plan<N, V>::TPL;
+  if constexpr (P < NP) {
+    if constexpr (P > 0) {
+      constexpr int RP = S::radix(P - 1), LP = S::Lbefore(P - 1);
+      __syncthreads();  // previous users of LDS are done
+      if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r)
-        lds[la.at(pass_out<N, S::radix(0), 1>(tid, r / S::radix(0), r % S::radix(0)))] = v[r];
-    }
-    __syncthreads();
-    if (active) {
+        for (int r = 0; r < E; ++r) lds[la.at(pass_out<N, TPL, RP, LP>(tid, r / RP, r % RP))] = v[r];
+      }
+      __syncthreads();
+      if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = lds[la.at(bpos<N, S::radix(1)>(tid, r))];
+        for (int r = 0; r < E; ++r) v[r] = lds[la.at(bpos<N, TPL, S::radix(P)>(tid, r))];
+      }
     }
-    pass_compute<T, N, S::radix(1), S::Lbefore(1), SIGN>(v, tid, tw);
+    pass_compute<T, N, E, S::radix(P), S::Lbefore(P), SIGN>(v, tid, tw);
+    fft_from_pass<T, N, V, ORD, SIGN, P + 1>(v, tid, tw, lds, la, active);
   }
-  if constexpr (NP > 2) {
-    __syncthreads();
-    if (active) {
-#pragma unroll
-      for (int r = 0; r < E; ++r)
-        lds[la.at(pass_out<N, S::radix(1), S::Lbefore(1)>(tid, r / S::radix(1), r % S::radix(1)))] = v[r];
-    }
-    __syncthreads();
-    if (active) {
-#pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = lds[la.at(bpos<N, S::radix(2)>(tid, r))];
-    }
-    pass_compute<T, N, S::radix(2), S::Lbefore(2), SIGN>(v, tid, tw);
-  }
-  static_assert(NP <= 3, "extend fft_regs for more passes");
+}
+
+template <typename T, int N, int V, int ORD, int SIGN, typename LA>
+__device__ __forceinline__ void fft_regs(cx<T>* v, int tid, const cx<T>* __restrict__ tw, cx<T>* lds, const LA& la,
+                                         bool active) {
+  fft_from_pass<T, N, V, ORD, SIGN, 0>(v, tid, tw, lds, la, active);
 }
 
 }  // namespace gpue

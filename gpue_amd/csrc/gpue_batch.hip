@@ -1,0 +1,330 @@
+// gpue_batch.hip -- many independent problems advanced together.
+//
+// Instances with the same shape (precision, dimensions, grid sizes, kernel
+// variants, gauge/no gauge) form a group.  For every sweep of the step one
+// batched kernel covers every instance of the group: blocks [i*bpi, (i+1)*bpi)
+// work on instance i, whose SweepArgs live in a device array.  GPUE's gauge
+// order alternates with the step parity (src/evolution.cu:837,842), so each
+// group has an even-step and an odd-step sequence; instances whose own step
+// counters differ in parity run in the other sequence of the same step (their
+// entries are inactive in the first).  Two consecutive steps of all groups are
+// captured once per gstate as a HIP graph and replayed.  Device memory is
+// allocated only in gpue_batch_create.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "ctx_internal.h"
+
+using namespace gpue;
+
+namespace {
+
+struct BSeq {                       // one (gstate, schedule parity, phase) sequence of a group
+  std::vector<Sweep> sweeps;
+  std::vector<SweepArgs*> dargs;   // per sweep: device array [members]
+  std::vector<FinArgs*> dfin;      // per sweep: device array [members] or null
+  std::vector<std::vector<SweepArgs>> hargs;
+  std::vector<std::vector<FinArgs>> hfin;
+};
+
+struct Group {
+  std::vector<int> members;
+  gpue_ctx* proto;                 // first member: shape and variants
+  BSeq seq[2][2][2];                // [gstate][schedule parity][phase]
+};
+
+bool same_shape(const gpue_ctx* a, const gpue_ctx* b) {
+  return a->prec == b->prec && a->p.dimnum == b->p.dimnum && a->dims[0] == b->dims[0] && a->dims[1] == b->dims[1] &&
+         a->dims[2] == b->dims[2] && a->var[0] == b->var[0] && a->var[1] == b->var[1] && a->var[2] == b->var[2] &&
+         a->p.corotating == b->p.corotating && a->az_zero == b->az_zero && a->p.device == b->p.device;
+}
+
+}  // namespace
+
+struct gpue_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<gpue_ctx*> ctx;
+  std::vector<int> active;
+  std::vector<int> cls;            // parity class: the instance's step parity at phase 0
+  std::vector<Group> groups;
+  int phase = 0;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::string err;
+};
+
+static gpue_status bfail(gpue_batch* b, gpue_status s, const std::string& m) {
+  if (b) b->err = m;
+  return s;
+}
+
+#define BHIP(b, expr)                                                                           \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) return bfail((b), GPUE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Fill the host argument arrays of every sequence from the members' current state.
+static void fill_args(gpue_batch* b) {
+  for (Group& g : b->groups) {
+    for (int gs = 0; gs < 2; ++gs)
+      for (int par = 0; par < 2; ++par)
+        for (int ph = 0; ph < 2; ++ph) {
+          BSeq& s = g.seq[gs][par][ph];
+          for (size_t k = 0; k < s.sweeps.size(); ++k) {
+            const Sweep& sw = s.sweeps[k];
+            for (size_t m = 0; m < g.members.size(); ++m) {
+              const int idx = g.members[m];
+              gpue_ctx* c = b->ctx[idx];
+              // the instance runs this sequence when its step parity at this phase is `par`
+              const int on = b->active[idx] && ((b->cls[idx] + ph) % 2 == par);
+              SweepArgs& a = s.hargs[k][m];
+              std::memset(&a, 0, sizeof(a));
+              a.active = on;
+              a.in = c->psi;
+              a.out = c->psi;
+              a.tw = c->tw[c->dims[sw.axis]];
+              a.ph = make_phys(c, gs, 0.0);
+              a.lines = c->dims[1] * c->dims[2];
+              a.axis = sw.axis;
+              a.gauge_axis = sw.axis;
+              a.flags = sw.flags;
+              a.invN = sw.invN;
+              a.norm_in = c->results;
+              a.partials = c->partials;
+              if (s.dfin[k]) {
+                FinArgs& f = s.hfin[k][m];
+                const int n = c->dims[sw.axis];
+                const int var = c->var[sw.axis];
+                f.active = on;
+                f.nb = sw.row ? sweep_blocks(true, n, var, c->dims[1] * c->dims[2], 1)
+                              : sweep_blocks(false, n, var, c->dims[0], sw.axis == 1 ? c->dims[2] : c->dims[1]);
+                f.na = 1;
+                f.mode = 1;
+                f.scale = c->d[0] * c->d[1] * c->d[2];
+                f.partials = c->partials;
+                f.out = c->results;
+              }
+            }
+          }
+        }
+  }
+}
+
+static gpue_status upload_args(gpue_batch* b) {
+  for (Group& g : b->groups)
+    for (int gs = 0; gs < 2; ++gs)
+      for (int par = 0; par < 2; ++par)
+        for (int ph = 0; ph < 2; ++ph) {
+          BSeq& s = g.seq[gs][par][ph];
+          for (size_t k = 0; k < s.sweeps.size(); ++k) {
+            BHIP(b, hipMemcpy(s.dargs[k], s.hargs[k].data(), s.hargs[k].size() * sizeof(SweepArgs),
+                              hipMemcpyHostToDevice));
+            if (s.dfin[k])
+              BHIP(b, hipMemcpy(s.dfin[k], s.hfin[k].data(), s.hfin[k].size() * sizeof(FinArgs),
+                                hipMemcpyHostToDevice));
+          }
+        }
+  return GPUE_OK;
+}
+
+// One step (one phase) of all groups for gstate.
+static gpue_status launch_phase(gpue_batch* b, int gs, int ph) {
+  for (Group& g : b->groups) {
+    gpue_ctx* c = g.proto;
+    for (int par = 0; par < 2; ++par) {
+      BSeq& s = g.seq[gs][par][ph];
+      for (size_t k = 0; k < s.sweeps.size(); ++k) {
+        const Sweep& sw = s.sweeps[k];
+        const int n = c->dims[sw.axis];
+        const int lx = sw.row ? c->dims[1] * c->dims[2] : c->dims[0];
+        const int others = sw.row ? 1 : (sw.axis == 1 ? c->dims[2] : c->dims[1]);
+        bool handled = false;
+        hipError_t e = launch_sweep_batched(c->prec, sw.row, n, c->var[sw.axis], sw.mode, s.dargs[k],
+                                            (int)g.members.size(), lx, others, b->stream, &handled);
+        if (!handled || e != hipSuccess) return bfail(b, GPUE_EHIP, "batched sweep launch");
+        if (s.dfin[k]) {
+          e = launch_finalize_batched(s.dfin[k], (int)g.members.size(), b->stream);
+          if (e != hipSuccess) return bfail(b, GPUE_EHIP, "batched finalize launch");
+        }
+      }
+    }
+  }
+  return GPUE_OK;
+}
+
+static void drop_batch_graphs(gpue_batch* b) {
+  for (int g = 0; g < 2; ++g) {
+    if (b->gexec[g]) (void)hipGraphExecDestroy(b->gexec[g]);
+    if (b->graph[g]) (void)hipGraphDestroy(b->graph[g]);
+    b->gexec[g] = nullptr;
+    b->graph[g] = nullptr;
+  }
+}
+
+extern "C" {
+
+void gpue_batch_destroy(gpue_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_batch_graphs(b);
+  for (Group& g : b->groups)
+    for (int gs = 0; gs < 2; ++gs)
+      for (int par = 0; par < 2; ++par)
+        for (int ph = 0; ph < 2; ++ph) {
+          BSeq& s = g.seq[gs][par][ph];
+          for (auto* p : s.dargs)
+            if (p) (void)hipFree(p);
+          for (auto* p : s.dfin)
+            if (p) (void)hipFree(p);
+        }
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* gpue_batch_last_error(const gpue_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
+  if (!ctxs || n <= 0 || !out) return GPUE_EINVAL;
+  *out = nullptr;
+  gpue_batch* b = new gpue_batch();
+  for (int i = 0; i < n; ++i) {
+    gpue_ctx* c = ctxs[i];
+    if (!c || c->v_time || c->p.device != ctxs[0]->p.device) {
+      gpue_batch_destroy(b);
+      return GPUE_EINVAL;   // time-dependent V and mixed devices are run per context
+    }
+    b->ctx.push_back(c);
+    b->active.push_back(1);
+    b->cls.push_back(c->next_step % 2);
+  }
+  b->device = ctxs[0]->p.device;
+  if (hipSetDevice(b->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    gpue_batch_destroy(b);
+    return GPUE_EHIP;
+  }
+  for (int i = 0; i < n; ++i) {
+    bool placed = false;
+    for (Group& g : b->groups)
+      if (same_shape(g.proto, b->ctx[i])) {
+        g.members.push_back(i);
+        placed = true;
+        break;
+      }
+    if (!placed) {
+      Group g;
+      g.proto = b->ctx[i];
+      g.members.push_back(i);
+      b->groups.push_back(g);
+    }
+  }
+  for (Group& g : b->groups)
+    for (int gs = 0; gs < 2; ++gs)
+      for (int par = 0; par < 2; ++par)
+        for (int ph = 0; ph < 2; ++ph) {
+          BSeq& s = g.seq[gs][par][ph];
+          // imaginary time: the norm slot holds 1.0 whenever nothing is pending, so
+          // the renormalising first sweep is always safe
+          schedule(g.proto, gs, par, gs == 0, s.sweeps);
+          const size_t M = g.members.size();
+          s.hargs.assign(s.sweeps.size(), std::vector<SweepArgs>(M));
+          s.hfin.assign(s.sweeps.size(), std::vector<FinArgs>(M));
+          s.dargs.assign(s.sweeps.size(), nullptr);
+          s.dfin.assign(s.sweeps.size(), nullptr);
+          for (size_t k = 0; k < s.sweeps.size(); ++k) {
+            if (hipMalloc((void**)&s.dargs[k], M * sizeof(SweepArgs)) != hipSuccess) {
+              gpue_batch_destroy(b);
+              return GPUE_ENOMEM;
+            }
+            if ((s.sweeps[k].flags & F_NORM) && hipMalloc((void**)&s.dfin[k], M * sizeof(FinArgs)) != hipSuccess) {
+              gpue_batch_destroy(b);
+              return GPUE_ENOMEM;
+            }
+          }
+        }
+  fill_args(b);
+  if (upload_args(b) != GPUE_OK) {
+    gpue_batch_destroy(b);
+    return GPUE_EHIP;
+  }
+  *out = b;
+  return GPUE_OK;
+}
+
+gpue_status gpue_batch_set_active(gpue_batch* b, int index, int active) {
+  if (!b || index < 0 || index >= (int)b->ctx.size()) return GPUE_EINVAL;
+  b->active[index] = active ? 1 : 0;
+  fill_args(b);
+  return upload_args(b);   // graphs keep their pointers; only the contents change
+}
+
+// Advance every active instance by nsteps steps of gstate.
+gpue_status gpue_batch_evolve(gpue_batch* b, int nsteps, int gstate) {
+  if (!b || nsteps < 0 || (gstate != 0 && gstate != 1)) return GPUE_EINVAL;
+  // settle per-instance state on the instances' own streams first
+  bool changed = false;
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    gpue_ctx* c = b->ctx[i];
+    if (!b->active[i]) continue;
+    if (gstate == 1 && c->pending_norm) {
+      if (apply_pending(c) != GPUE_OK) return bfail(b, GPUE_EHIP, c->err);
+    }
+    BHIP(b, hipStreamSynchronize(c->stream));
+    const int cls = (c->next_step + b->phase) % 2;   // parity at phase 0 of the current pair
+    if (cls != b->cls[i]) {
+      b->cls[i] = cls;
+      changed = true;
+    }
+  }
+  if (changed) {
+    fill_args(b);
+    gpue_status st = upload_args(b);
+    if (st != GPUE_OK) return st;
+  }
+  int s = 0;
+  if (b->phase == 1 && s < nsteps) {   // finish a pair started by an odd call
+    gpue_status st = launch_phase(b, gstate, 1);
+    if (st != GPUE_OK) return st;
+    b->phase = 0;
+    ++s;
+  }
+  if (nsteps - s >= 2) {
+    if (!b->gexec[gstate]) {
+      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      gpue_status st = launch_phase(b, gstate, 0);
+      if (st == GPUE_OK) st = launch_phase(b, gstate, 1);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (st != GPUE_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (e != hipSuccess) return bfail(b, GPUE_EHIP, "batch graph capture");
+      if (hipGraphInstantiate(&b->gexec[gstate], g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return bfail(b, GPUE_EHIP, "batch graph instantiate");
+      }
+      b->graph[gstate] = g;
+    }
+    for (; nsteps - s >= 2; s += 2) BHIP(b, hipGraphLaunch(b->gexec[gstate], b->stream));
+  }
+  if (s < nsteps) {
+    gpue_status st = launch_phase(b, gstate, 0);
+    if (st != GPUE_OK) return st;
+    b->phase = 1;
+    ++s;
+  }
+  BHIP(b, hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    if (!b->active[i]) continue;
+    b->ctx[i]->next_step += nsteps;
+    if (gstate == 0) b->ctx[i]->pending_norm = true;
+  }
+  return GPUE_OK;
+}
+
+}  // extern "C"

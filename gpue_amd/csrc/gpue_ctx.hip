@@ -14,9 +14,8 @@
 #include <string>
 #include <vector>
 
+#include "ctx_internal.h"
 #include "eqn.h"
-#include "gpue_amd.h"
-#include "misc_kernels.h"
 
 namespace gpue {
 void make_grids(int dimnum, int xDim, int yDim, int zDim, double xMax, double yMax, double zMax,
@@ -27,69 +26,10 @@ void twiddles(int n, std::vector<double>& re, std::vector<double>& im);
 
 using namespace gpue;
 
-namespace {
-
-struct Sweep {
-  bool row;
-  int axis;
-  int mode;
-  unsigned flags;
-  double invN;
-};
-
-constexpr int TIMING_PAIRS = 256;
-
-}  // namespace
-
-struct gpue_ctx {
-  gpue_params p;
-  int prec = 64;
-  size_t esz = 16;   // bytes per complex element
-  long N = 0;
-  int dims[3] = {1, 1, 1};
-  double d[3] = {1, 1, 1};
-  std::vector<double> grid[6];
-  hipStream_t stream = nullptr;
-  void* psi = nullptr;
-  void* tmp = nullptr;
-  double* dgrid[6] = {};
-  std::map<int, void*> tw;   // N -> twiddle table in the working precision
-  double* partials = nullptr;
-  size_t partials_n = 0;
-  double* results = nullptr;   // [0] norm  [1] kinetic  [2..5] position terms
-  int v_kind = V_SEP, a_kind = A_ROTATION, k_kind = K_SEP;
-  bool v_user = false;         // V supplied by the caller (table or equation)
-  double* vs[3] = {};          // separable V / K value tables
-  double* ks[3] = {};
-  double* Vt = nullptr;
-  double* Kt = nullptr;
-  double* At[3] = {};
-  bool az_zero = true;
-  bool v_time = false;         // V is a time-dependent equation, refreshed each step
-  EqnProg veq;
-  bool pending_norm = false;
-  double last_energy = 0.0;
-  std::string err;
-  // vortex scratch
-  unsigned char* vmask = nullptr;
-  double vmask_radius = -1.0;
-  int* vcand = nullptr;
-  int* vmarker = nullptr;
-  int* vfound = nullptr;
-  // timing
-  bool timing = false;
-  std::vector<hipEvent_t> ev;
-  std::vector<int> ev_kind;
-  int ev_used = 0;
-  std::vector<std::string> kind_names;
-  std::vector<long> kind_n;
-  std::vector<double> kind_ms;
-};
-
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-static gpue_status fail(gpue_ctx* c, gpue_status s, const std::string& msg) {
+gpue_status gpue::fail(gpue_ctx* c, gpue_status s, const std::string& msg) {
   if (c) c->err = msg;
   return s;
 }
@@ -109,7 +49,7 @@ static double* dev_copy(const std::vector<double>& v, hipError_t* e) {
   return p;
 }
 
-static Phys make_phys(const gpue_ctx* c, int gstate, double time) {
+Phys gpue::make_phys(const gpue_ctx* c, int gstate, double time) {
   const gpue_params& p = c->p;
   Phys f;
   std::memset(&f, 0, sizeof(f));
@@ -144,6 +84,7 @@ static Phys make_phys(const gpue_ctx* c, int gstate, double time) {
     (&f.pxg)[a] = c->dgrid[a + 3];
     f.vs[a] = c->vs[a];
     f.ks[a] = c->ks[a];
+    f.kf[a] = c->kf[gstate][a];
   }
   f.Vt = c->Vt;
   f.Kt = c->Kt;
@@ -153,7 +94,7 @@ static Phys make_phys(const gpue_ctx* c, int gstate, double time) {
 
 // Sweeps of step i (src/evolution.cu:727-870).  NORM is set on the last sweep
 // of an imaginary-time step; SCALE_IN on the first when a norm is pending.
-static void schedule(const gpue_ctx* c, int gstate, int i, bool pending, std::vector<Sweep>& out) {
+void gpue::schedule(const gpue_ctx* c, int gstate, int i, bool pending, std::vector<Sweep>& out) {
   out.clear();
   const gpue_params& p = c->p;
   const int nx = c->dims[0], ny = c->dims[1], nz = c->dims[2];
@@ -222,10 +163,12 @@ static gpue_status timing_flush(gpue_ctx* c) {
   return GPUE_OK;
 }
 
-static gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void* in, void* out) {
+gpue_status gpue::launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void* in, void* out) {
   const int n = c->dims[s.axis];
+  const int var = s.var >= 0 ? s.var : c->var[s.axis];
   SweepArgs a;
   std::memset(&a, 0, sizeof(a));
+  a.active = 1;
   a.in = in;
   a.out = out;
   a.tw = c->tw[n];
@@ -248,15 +191,15 @@ static gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const voi
     HIPCHK(c, hipEventRecord(c->ev[2 * slot], c->stream));
   }
   bool handled = false;
-  hipError_t e = launch_sweep(c->prec, s.row, n, s.mode, a, c->stream, &handled);
+  hipError_t e = launch_sweep(c->prec, s.row, n, var, s.mode, a, c->stream, &handled);
   if (!handled) return fail(c, GPUE_EUNSUPPORTED, "no sweep kernel for n=" + std::to_string(n));
   if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("sweep launch: ") + hipGetErrorString(e));
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2 * slot + 1], c->stream));
   // second stage of the fused reductions
   const bool ksum = s.mode == (s.row ? (int)RM_KSUM : (int)CM_KSUM);
   if (ksum || (s.flags & F_NORM)) {
-    const int nb = s.row ? sweep_blocks(true, n, c->dims[1] * c->dims[2], 1)
-                         : sweep_blocks(false, n, c->dims[0], s.axis == 1 ? c->dims[2] : c->dims[1]);
+    const int nb = s.row ? sweep_blocks(true, n, var, c->dims[1] * c->dims[2], 1)
+                         : sweep_blocks(false, n, var, c->dims[0], s.axis == 1 ? c->dims[2] : c->dims[1]);
     const double dV = c->d[0] * c->d[1] * c->d[2];
     e = ksum ? launch_finalize(c->partials, nb, 1, 0, s.invN, c->results + 1, c->stream)
              : launch_finalize(c->partials, nb, 1, 1, dV, c->results + 0, c->stream);
@@ -265,16 +208,27 @@ static gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const voi
   return GPUE_OK;
 }
 
-static gpue_status apply_pending(gpue_ctx* c) {
+gpue_status gpue::apply_pending(gpue_ctx* c) {
   if (!c->pending_norm) return GPUE_OK;
   hipError_t e = c->prec == 64 ? launch_apply_norm<double>(c->psi, c->N, c->results, c->stream)
                                : launch_apply_norm<float>(c->psi, c->N, c->results, c->stream);
+  if (e == hipSuccess) e = launch_set_double(c->results, 1.0, c->stream);   // nothing pending: norm 1
   if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
   c->pending_norm = false;
   return GPUE_OK;
 }
 
 static bool valid_dim(int n) { return sweep_supported(n); }
+
+// Any change of fields, parameters or kernel variants invalidates the captured graphs.
+static void drop_graphs(gpue_ctx* c) {
+  for (int g = 0; g < 2; ++g) {
+    if (c->gexec[g]) (void)hipGraphExecDestroy(c->gexec[g]);
+    if (c->graph[g]) (void)hipGraphDestroy(c->graph[g]);
+    c->gexec[g] = nullptr;
+    c->graph[g] = nullptr;
+  }
+}
 
 // Per-axis terms of kharmonic_V (src/operators.cu:994-1008) with the rotation
 // gauge's 1/2 m |A|^2 (Ax = -y omega omegaX, Ay = x omega omegaY, Az = 0,
@@ -369,14 +323,20 @@ void gpue_destroy(gpue_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* bufs[] = {c->psi, c->tmp, c->partials, c->results, c->Vt, c->Kt, c->At[0], c->At[1], c->At[2],
-                  c->vs[0], c->vs[1], c->vs[2], c->ks[0], c->ks[1], c->ks[2],
+                  c->vs[0], c->vs[1], c->vs[2], c->ks[0], c->ks[1], c->ks[2], c->kf[0][0], c->kf[0][1],
+                  c->kf[0][2], c->kf[1][0], c->kf[1][1], c->kf[1][2],
                   c->vmask, c->vcand, c->vmarker, c->vfound};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int a = 0; a < 6; ++a)
     if (c->dgrid[a]) (void)hipFree(c->dgrid[a]);
   for (auto& kv : c->tw) (void)hipFree(kv.second);
-  for (auto& e : c->ev) (void)hipEventDestroy(e);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (int g = 0; g < 2; ++g) {
+    if (c->gexec[g]) (void)hipGraphExecDestroy(c->gexec[g]);
+    if (c->graph[g]) (void)hipGraphDestroy(c->graph[g]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -405,7 +365,7 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
   for (int a = 0; a < p.dimnum; ++a)
     if (!valid_dim(c->dims[a]))
       return bail(GPUE_EUNSUPPORTED, "unsupported grid size " + std::to_string(c->dims[a]) +
-                                         " (supported: 16 32 64 128 192 256 384 512 1024)");
+                                         " (supported: 16 32 48 64 96 128 192 256 384 512 1024)");
   if (!(p.hbar > 0) || !(p.mass > 0) || !(p.xMax > 0) || (p.dimnum > 1 && !(p.yMax > 0)) ||
       (p.dimnum > 2 && !(p.zMax > 0)))
     return bail(GPUE_EINVAL, "hbar, mass and box sizes must be positive");
@@ -433,6 +393,18 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
     if (e != hipSuccess) return bail(GPUE_ENOMEM, "V tables");
     c->ks[a] = dev_copy(ksh[a], &e);
     if (e != hipSuccess) return bail(GPUE_ENOMEM, "K tables");
+    // K operator factors: exp(-K_a gdt/hbar) and exp(-i K_a dt/hbar) (src/operators.cu:1082,1102-1103)
+    std::vector<double> fg(2 * ksh[a].size()), fe(2 * ksh[a].size());
+    for (size_t i = 0; i < ksh[a].size(); ++i) {
+      fg[2 * i] = std::exp(-ksh[a][i] * (p.gdt / p.hbar));
+      fg[2 * i + 1] = 0.0;
+      fe[2 * i] = std::cos(-ksh[a][i] * (p.dt / p.hbar));
+      fe[2 * i + 1] = std::sin(-ksh[a][i] * (p.dt / p.hbar));
+    }
+    c->kf[0][a] = dev_copy(fg, &e);
+    if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
+    c->kf[1][a] = dev_copy(fe, &e);
+    if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
   }
   for (int a = 0; a < p.dimnum; ++a) {
     int n = c->dims[a];
@@ -455,21 +427,23 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
   }
   // reduction scratch: the largest grid of any reducing kernel, times 4 accumulators
   size_t nb = REDUCE_BLOCKS;
-  nb = std::max<size_t>(nb, sweep_blocks(true, p.xDim, p.yDim * p.zDim, 1));
-  if (p.dimnum > 1) nb = std::max<size_t>(nb, sweep_blocks(false, p.yDim, p.xDim, p.zDim));
-  if (p.dimnum > 2) nb = std::max<size_t>(nb, sweep_blocks(false, p.zDim, p.xDim, p.yDim));
+  for (int v = 0; v < N_VARIANTS; ++v) {
+    nb = std::max<size_t>(nb, sweep_blocks(true, p.xDim, v, p.yDim * p.zDim, 1));
+    if (p.dimnum > 1) nb = std::max<size_t>(nb, sweep_blocks(false, p.yDim, v, p.xDim, p.zDim));
+    if (p.dimnum > 2) nb = std::max<size_t>(nb, sweep_blocks(false, p.zDim, v, p.xDim, p.yDim));
+  }
+  // default variant: narrow lines for small (latency-bound) grids, wide ones otherwise
+  for (int a = 0; a < 3; ++a) c->var[a] = c->N <= (1L << 20) ? 1 : 0;
   c->partials_n = nb * 4;
   if (hipMalloc((void**)&c->partials, c->partials_n * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->results, 16 * sizeof(double)) != hipSuccess)
     return bail(GPUE_ENOMEM, "reduction scratch");
-  if (hipMemset(c->results, 0, 16 * sizeof(double)) != hipSuccess ||
-      hipMemset(c->psi, 0, c->N * c->esz) != hipSuccess)
+  // on the context's own stream: other contexts' work on the device is not waited for
+  if (hipMemsetAsync(c->results, 0, 16 * sizeof(double), c->stream) != hipSuccess ||
+      launch_set_double(c->results, 1.0, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->psi, 0, c->N * c->esz, c->stream) != hipSuccess)
     return bail(GPUE_EHIP, "memset");
-  c->ev.resize(2 * TIMING_PAIRS);
-  c->ev_kind.resize(TIMING_PAIRS);
-  for (auto& ev : c->ev)
-    if (hipEventCreate(&ev) != hipSuccess) return bail(GPUE_EHIP, "event create");
-  if (hipDeviceSynchronize() != hipSuccess) return bail(GPUE_EHIP, "device sync");
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(GPUE_EHIP, "stream sync");
   *out = c;
   return GPUE_OK;
 }
@@ -483,6 +457,7 @@ gpue_status gpue_get_grid(gpue_ctx* c, int axis, double* out) {
 gpue_status gpue_set_g(gpue_ctx* c, double g) {
   if (!c) return GPUE_EINVAL;
   c->p.g = g;
+  drop_graphs(c);
   return GPUE_OK;
 }
 
@@ -493,6 +468,7 @@ gpue_status gpue_sync(gpue_ctx* c) {
 }
 
 static gpue_status set_table(gpue_ctx* c, int which, const double* src, bool from_device) {
+  drop_graphs(c);
   double** slot = which == GPUE_FIELD_V ? &c->Vt : which == GPUE_FIELD_K ? &c->Kt : &c->At[which - GPUE_FIELD_AX];
   if (!*slot) {
     if (hipMalloc((void**)slot, c->N * sizeof(double)) != hipSuccess) return fail(c, GPUE_ENOMEM, "field table");
@@ -566,6 +542,7 @@ gpue_status gpue_set_field_eqn(gpue_ctx* c, int which, const char* expr, const c
     if (which != GPUE_FIELD_V)
       return fail(c, GPUE_EUNSUPPORTED, "time-dependent K and A fields are not supported (V only)");
     if (!c->Vt && hipMalloc((void**)&c->Vt, c->N * sizeof(double)) != hipSuccess) return fail(c, GPUE_ENOMEM, "V");
+    drop_graphs(c);
     c->veq = dp;
     c->v_kind = V_TABLE;
     c->v_user = true;
@@ -595,6 +572,7 @@ gpue_status gpue_upload_wfc(gpue_ctx* c, const double* il) {
     for (long i = 0; i < 2 * c->N; ++i) f[i] = (float)il[i];
     HIPCHK(c, hipMemcpyAsync(c->psi, f.data(), c->N * 8, hipMemcpyHostToDevice, c->stream));
   }
+  HIPCHK(c, launch_set_double(c->results, 1.0, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->pending_norm = false;
   return GPUE_OK;
@@ -630,8 +608,12 @@ gpue_status gpue_parsum(gpue_ctx* c) {
   return apply_pending(c);
 }
 
-gpue_status gpue_energy(gpue_ctx* c, double* energy, double* mu) {
-  if (!c) return GPUE_EINVAL;
+}  // extern "C"
+
+// Queue the energy terms (kinetic by Parseval, position terms, GPUE's
+// forward-difference gauge term) into results[1..5]; copy results[0..5] to
+// host_dst (pinned memory for an asynchronous caller) when given.
+gpue_status gpue::energy_launch(gpue_ctx* c, double* host_dst) {
   gpue_status st = apply_pending(c);
   if (st != GPUE_OK) return st;
   const gpue_params& p = c->p;
@@ -653,17 +635,93 @@ gpue_status gpue_energy(gpue_ctx* c, double* energy, double* mu) {
                                : launch_energy_pos<float>(c->psi, ph, gauge, p.hbar, c->d[0], c->d[1], c->d[2],
                                                           c->partials, &nb, c->stream);
   if (e == hipSuccess) e = launch_finalize(c->partials, nb, 4, 0, 1.0, c->results + 2, c->stream);
+  if (e == hipSuccess && host_dst)
+    e = hipMemcpyAsync(host_dst, c->results, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
-  double r[6];
-  HIPCHK(c, hipMemcpyAsync(r, c->results, sizeof(r), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GPUE_OK;
+}
+
+// E from the terms as src/split_op.cu:208-375 (energy_calc) sums them, with
+// g|psi|^4/2; the chemical potential counts the interaction term once more.
+void gpue::energy_combine(const gpue_ctx* c, const double* r, double* energy, double* mu) {
+  const gpue_params& p = c->p;
   const double dV = c->d[0] * c->d[1] * c->d[2];
   const double kin = r[1], vt = r[2], in4 = r[3], lz = r[4];
-  double E = p.gpe ? (kin + vt + 0.5 * p.g * in4 + lz) * dV : (kin + vt + lz) * dV;
-  double M = p.gpe ? (kin + vt + p.g * in4 + lz) * dV : (kin + vt + lz) * dV;
+  if (energy) *energy = p.gpe ? (kin + vt + 0.5 * p.g * in4 + lz) * dV : (kin + vt + lz) * dV;
+  if (mu) *mu = p.gpe ? (kin + vt + p.g * in4 + lz) * dV : (kin + vt + lz) * dV;
+}
+
+extern "C" {
+
+gpue_status gpue_energy(gpue_ctx* c, double* energy, double* mu) {
+  if (!c) return GPUE_EINVAL;
+  double r[6];
+  gpue_status st = energy_launch(c, nullptr);
+  if (st != GPUE_OK) return st;
+  HIPCHK(c, hipMemcpyAsync(r, c->results, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double E = 0, M = 0;
+  energy_combine(c, r, &E, &M);
   if (energy) *energy = E;
   if (mu) *mu = M;
   if (!std::isfinite(E)) return fail(c, GPUE_ENONFINITE, "energy is not finite");
+  return GPUE_OK;
+}
+
+// Autotuner: for every sweep of the imaginary- and real-time schedules, time
+// each kernel variant (out of place, psi -> tmp, so the state is untouched) and
+// keep, per axis, the variant with the smallest summed time.
+static gpue_status autotune(gpue_ctx* c, int reps) {
+  if (reps <= 0) reps = 5;
+  // the trial launches overwrite the reduction slots: settle a pending norm first
+  gpue_status st0 = apply_pending(c);
+  if (st0 != GPUE_OK) return st0;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  double cost[3][N_VARIANTS] = {};
+  std::vector<Sweep> sched;
+  const bool timing = c->timing;
+  c->timing = false;
+  for (int gstate = 0; gstate < 2; ++gstate) {
+    for (int parity = 0; parity < 2; ++parity) {
+      schedule(c, gstate, parity, true, sched);
+      Phys ph = make_phys(c, gstate, 0.0);
+      for (Sweep s : sched) {
+        for (int v = 0; v < N_VARIANTS; ++v) {
+          s.var = v;
+          gpue_status st = launch(c, s, ph, c->psi, c->tmp);   // warm-up
+          if (st != GPUE_OK) return st;
+          HIPCHK(c, hipEventRecord(e0, c->stream));
+          for (int r = 0; r < reps; ++r) {
+            st = launch(c, s, ph, c->psi, c->tmp);
+            if (st != GPUE_OK) return st;
+          }
+          HIPCHK(c, hipEventRecord(e1, c->stream));
+          HIPCHK(c, hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+          cost[s.axis][v] += ms;
+        }
+      }
+    }
+  }
+  c->timing = timing;
+  // the trial launches wrote their norms into results[0]: restore "nothing pending"
+  {
+    hipError_t e = launch_set_double(c->results, 1.0, c->stream);
+    if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
+  }
+  for (int a = 0; a < 3; ++a) {
+    int best = c->var[a];
+    for (int v = 0; v < N_VARIANTS; ++v)
+      if (cost[a][v] > 0 && cost[a][v] < cost[a][best]) best = v;
+    c->var[a] = best;
+  }
+  drop_graphs(c);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->tuned = true;
   return GPUE_OK;
 }
 
@@ -677,16 +735,77 @@ static gpue_status refresh_V(gpue_ctx* c, double t) {
   return GPUE_OK;
 }
 
-gpue_status gpue_evolve(gpue_ctx* c, int first_step, int nsteps, int gstate, int energy_every, double energy_thr,
-                        double* energies, int max_energies, gpue_evolve_result* res) {
-  if (!c || nsteps < 0 || (gstate != 0 && gstate != 1) || first_step < 0) return GPUE_EINVAL;
+// Capture graph_steps steady-state steps (starting on an even step, norm
+// pending in imaginary time) into a graph for this gstate.
+static gpue_status build_graph(gpue_ctx* c, int gstate) {
+  std::vector<Sweep> sched;
+  const double Dt = gstate == 0 ? c->p.gdt : c->p.dt;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  gpue_status st = GPUE_OK;
+  for (int s = 0; s < c->graph_steps && st == GPUE_OK; ++s) {
+    Phys ph = make_phys(c, gstate, Dt * s);
+    schedule(c, gstate, s, gstate == 0, sched);
+    for (const Sweep& sw : sched) {
+      st = launch(c, sw, ph, c->psi, c->psi);
+      if (st != GPUE_OK) break;
+    }
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (st != GPUE_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+  hipGraphExec_t ge = nullptr;
+  e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(c, GPUE_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  }
+  c->graph[gstate] = g;
+  c->gexec[gstate] = ge;
+  return GPUE_OK;
+}
+
+static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gstate, int energy_every,
+                               double energy_thr, double* energies, int max_energies, gpue_evolve_result* res,
+                               bool check_norm) {
   gpue_evolve_result r;
   std::memset(&r, 0, sizeof(r));
+  // real time starts from the renormalised state: settle a norm left pending
+  // by imaginary time (GPUE divides at the end of every imaginary step)
+  if (gstate == 1) {
+    gpue_status st = apply_pending(c);
+    if (st != GPUE_OK) return st;
+  }
   std::vector<Sweep> sched;
   const double Dt = gstate == 0 ? c->p.gdt : c->p.dt;
   int s;
+  const int G = c->graph_steps;
   for (s = 0; s < nsteps; ++s) {
     const int i = first_step + s;
+    // steady state: replay a captured block of G steps when no energy cadence
+    // falls inside it and the fields do not depend on time
+    // (the imaginary-time graph always rescales on entry: results[0] holds the
+    // pending norm or 1.0, and x*1.0 is exact)
+    if (G > 0 && !c->timing && !c->v_time && (i % 2) == 0 && nsteps - s >= G) {
+      bool cadence_inside = false;
+      if (energy_every > 0) {
+        const int nxt = ((i + energy_every - 1) / energy_every) * energy_every;
+        cadence_inside = nxt < i + G;
+      }
+      if (!cadence_inside) {
+        if (!c->gexec[gstate]) {
+          gpue_status st = build_graph(c, gstate);
+          if (st != GPUE_OK) return st;
+        }
+        HIPCHK(c, hipGraphLaunch(c->gexec[gstate], c->stream));
+        if (gstate == 0) c->pending_norm = true;
+        s += G - 1;
+        continue;
+      }
+    }
     Phys ph = make_phys(c, gstate, Dt * i);   // time = Dt*i (src/evolution.cu:408)
     if (c->v_time) {
       gpue_status st = refresh_V(c, Dt * i);
@@ -716,11 +835,12 @@ gpue_status gpue_evolve(gpue_ctx* c, int first_step, int nsteps, int gstate, int
     }
   }
   r.steps_done = s;
+  c->next_step = first_step + s;
   if (c->timing) {
     gpue_status st = timing_flush(c);
     if (st != GPUE_OK) return st;
   }
-  if (gstate == 0 && c->pending_norm) {
+  if (check_norm && gstate == 0 && c->pending_norm) {
     double nrm = 0;
     HIPCHK(c, hipMemcpyAsync(&nrm, c->results, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -729,6 +849,22 @@ gpue_status gpue_evolve(gpue_ctx* c, int first_step, int nsteps, int gstate, int
   if (res) *res = r;
   return GPUE_OK;
 }
+
+gpue_status gpue_evolve(gpue_ctx* c, int first_step, int nsteps, int gstate, int energy_every, double energy_thr,
+                        double* energies, int max_energies, gpue_evolve_result* res) {
+  if (!c || nsteps < 0 || (gstate != 0 && gstate != 1) || first_step < 0) return GPUE_EINVAL;
+  return evolve_impl(c, first_step, nsteps, gstate, energy_every, energy_thr, energies, max_energies, res, true);
+}
+
+}  // extern "C"
+
+// Queue nsteps steps without waiting for them (no energy cadence, no norm check).
+gpue_status gpue::evolve_async(gpue_ctx* c, int first_step, int nsteps, int gstate) {
+  if (!c || nsteps < 0 || (gstate != 0 && gstate != 1) || first_step < 0) return GPUE_EINVAL;
+  return evolve_impl(c, first_step, nsteps, gstate, 0, 0.0, nullptr, 0, nullptr, false);
+}
+
+extern "C" {
 
 gpue_status gpue_vortex_marker_2d(gpue_ctx* c, double radius, int* marker_out, int* found) {
   if (!c || !marker_out) return GPUE_EINVAL;
@@ -788,6 +924,11 @@ gpue_status gpue_vortex_list_2d(gpue_ctx* c, double radius, int* ijw, int max_v,
 
 gpue_status gpue_set_timing(gpue_ctx* c, int on) {
   if (!c) return GPUE_EINVAL;
+  if (on && c->ev.empty()) {   // event pairs are created on first use
+    c->ev_kind.resize(TIMING_PAIRS);
+    c->ev.assign(2 * TIMING_PAIRS, nullptr);
+    for (auto& ev : c->ev) HIPCHK(c, hipEventCreate(&ev));
+  }
   c->timing = on != 0;
   c->ev_used = 0;
   c->kind_names.clear();
@@ -835,6 +976,31 @@ gpue_status gpue_eqn_parse(const char* expr, const char* const* names, const dou
 
 double gpue_eqn_eval(const gpue_eqn_node* nodes, int n, double x, double y, double z, double t) {
   return eqn_eval(reinterpret_cast<const EqnNodeC*>(nodes), n, x, y, z, t);
+}
+
+gpue_status gpue_set_graph_steps(gpue_ctx* c, int steps) {
+  if (!c || steps < 0 || (steps % 2) != 0) return GPUE_EINVAL;
+  c->graph_steps = steps;
+  drop_graphs(c);
+  return GPUE_OK;
+}
+
+gpue_status gpue_autotune(gpue_ctx* c, int reps, int* variants) {
+  if (!c) return GPUE_EINVAL;
+  gpue_status st = autotune(c, reps);
+  if (variants)
+    for (int a = 0; a < 3; ++a) variants[a] = c->var[a];
+  return st;
+}
+
+gpue_status gpue_set_variants(gpue_ctx* c, const int* variants) {
+  if (!c || !variants) return GPUE_EINVAL;
+  for (int a = 0; a < 3; ++a) {
+    if (variants[a] < 0 || variants[a] >= N_VARIANTS) return fail(c, GPUE_EINVAL, "variant out of range");
+  }
+  for (int a = 0; a < 3; ++a) c->var[a] = variants[a];
+  drop_graphs(c);
+  return GPUE_OK;
 }
 
 }  // extern "C"

@@ -22,8 +22,23 @@ hipError_t launch_vortex(const void* psi, const unsigned char* mask, int xDim, i
                          hipStream_t s);
 
 // sweep dispatch (sweep_inst_*.hip)
-hipError_t launch_sweep(int prec, bool row, int N, int mode, const SweepArgs& a, hipStream_t s, bool* handled);
-int sweep_blocks(bool row, int N, int lines_or_xdim, int others);
+hipError_t launch_sweep(int prec, bool row, int N, int var, int mode, const SweepArgs& a, hipStream_t s,
+                        bool* handled);
+int sweep_blocks(bool row, int N, int var, int lines_or_xdim, int others);
+hipError_t launch_sweep_batched(int prec, bool row, int N, int var, int mode, const SweepArgs* d, int ninst, int lx,
+                                int others, hipStream_t s, bool* handled);
+
+// Second stage of a batched reduction: one workgroup per instance.
+struct FinArgs {
+  int active;
+  int nb, na, mode;
+  double scale;
+  const double* partials;
+  double* out;
+};
+hipError_t launch_finalize_batched(const FinArgs* d, int ninst, hipStream_t s);
+hipError_t launch_set_double(double* p, double v, hipStream_t s);
+constexpr int N_VARIANTS = 2;
 bool sweep_supported(int N);
 
 }  // namespace gpue

@@ -77,7 +77,7 @@ __global__ void k_norm_sum(const cx<T>* psi, long N, double* partials) {
     acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
   }
   double pv[1] = {acc};
-  block_partial<1>(pv, partials, sh);
+  block_partial<1>(pv, partials, sh, blockIdx.x, gridDim.x);
 }
 
 // Second stage of every reduction: one workgroup sums nb partials of each of
@@ -142,7 +142,7 @@ __global__ void k_energy_pos(const cx<T>* psi, Phys p, int gauge, double hbar, d
       acc[2] += wx * (-syy * hbar) - (-wy) * (sx * hbar);
     }
   }
-  block_partial<4>(acc, partials, sh);
+  block_partial<4>(acc, partials, sh, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -222,6 +222,15 @@ __global__ void k_vortex_resolve(const int* cand, int xDim, int* marker, int* fo
   if (local) atomicAdd(found, local);
 }
 
+__global__ void k_set_double(double* p, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+
+hipError_t launch_set_double(double* p, double v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, s, p, v);
+  return hipGetLastError();
+}
+
 __global__ void k_fill_int(int* p, long n, int v) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -254,6 +263,24 @@ template <typename T>
 hipError_t launch_norm_sum(const void* psi, long N, double* partials, int* nb, hipStream_t s) {
   *nb = gs_blocks(N, 256, REDUCE_BLOCKS);
   hipLaunchKernelGGL(k_norm_sum<T>, dim3(*nb), dim3(256), 0, s, (const cx<T>*)psi, N, partials);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_finalize_b(const FinArgs* __restrict__ args) {
+  const FinArgs& f = args[blockIdx.x];
+  if (!f.active) return;
+  __shared__ double sh[64];
+  for (int a = 0; a < f.na; ++a) {
+    double acc[1] = {0.0};
+    for (int i = threadIdx.x; i < f.nb; i += blockDim.x) acc[0] += f.partials[(size_t)a * f.nb + i];
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) f.out[a] = f.mode == 1 ? sqrt(acc[0] * f.scale) : acc[0] * f.scale;
+    __syncthreads();
+  }
+}
+
+hipError_t launch_finalize_batched(const FinArgs* d, int ninst, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize_b, dim3(ninst), dim3(256), 0, s, d);
   return hipGetLastError();
 }
 

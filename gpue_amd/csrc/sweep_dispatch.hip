@@ -1,40 +1,67 @@
-// sweep_dispatch.hip -- maps (precision, N) to the per-N translation units.
+// sweep_dispatch.hip -- maps (precision, N, variant) to the per-N translation units.
 #include "misc_kernels.h"
 #include "sweep_launch.h"
 
 namespace gpue {
 
-#define GPUE_DECL(N_)                                                                   \
-  hipError_t sweep_f64_##N_(bool row, int mode, const SweepArgs& a, hipStream_t s);     \
-  hipError_t sweep_f32_##N_(bool row, int mode, const SweepArgs& a, hipStream_t s);
-GPUE_DECL(16) GPUE_DECL(32) GPUE_DECL(64) GPUE_DECL(128) GPUE_DECL(192) GPUE_DECL(256)
+#define GPUE_DECL1(NAME_)                                                                         \
+  hipError_t NAME_(bool row, int mode, const SweepArgs& a, hipStream_t s);                        \
+  hipError_t NAME_##_b(bool row, int mode, const SweepArgs* d, int ninst, int lx, int others, hipStream_t s);
+#define GPUE_DECL(N_)                                                                     \
+  GPUE_DECL1(sweep_f64_##N_##_v0) GPUE_DECL1(sweep_f32_##N_##_v0)                         \
+  GPUE_DECL1(sweep_f64_##N_##_v1) GPUE_DECL1(sweep_f32_##N_##_v1)
+GPUE_DECL(16) GPUE_DECL(32) GPUE_DECL(48) GPUE_DECL(64) GPUE_DECL(96) GPUE_DECL(128) GPUE_DECL(192) GPUE_DECL(256)
 GPUE_DECL(384) GPUE_DECL(512) GPUE_DECL(1024)
 #undef GPUE_DECL
 
 bool sweep_supported(int N) {
   switch (N) {
-    case 16: case 32: case 64: case 128: case 192: case 256: case 384: case 512: case 1024: return true;
+    case 16: case 32: case 48: case 64: case 96: case 128: case 192: case 256: case 384: case 512: case 1024:
+      return true;
     default: return false;
   }
 }
 
-hipError_t launch_sweep(int prec, bool row, int N, int mode, const SweepArgs& a, hipStream_t s, bool* handled) {
+hipError_t launch_sweep(int prec, bool row, int N, int var, int mode, const SweepArgs& a, hipStream_t s,
+                        bool* handled) {
   *handled = true;
-#define GPUE_CASE(N_) \
-  case N_: return prec == 64 ? sweep_f64_##N_(row, mode, a, s) : sweep_f32_##N_(row, mode, a, s);
+#define GPUE_CASE(N_)                                                                         \
+  case N_:                                                                                    \
+    if (var == 0) return prec == 64 ? sweep_f64_##N_##_v0(row, mode, a, s) : sweep_f32_##N_##_v0(row, mode, a, s); \
+    return prec == 64 ? sweep_f64_##N_##_v1(row, mode, a, s) : sweep_f32_##N_##_v1(row, mode, a, s);
   switch (N) {
-    GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(64) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
+    GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
     GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
     default: *handled = false; return hipErrorInvalidValue;
   }
 #undef GPUE_CASE
 }
 
-int sweep_blocks(bool row, int N, int lines_or_xdim, int others) {
-#define GPUE_CASE(N_) \
-  case N_: return row ? row_blocks<N_>(lines_or_xdim) : col_blocks<N_>(lines_or_xdim, others);
+hipError_t launch_sweep_batched(int prec, bool row, int N, int var, int mode, const SweepArgs* d, int ninst, int lx,
+                                int others, hipStream_t s, bool* handled) {
+  *handled = true;
+#define GPUE_CASE(N_)                                                                                  \
+  case N_:                                                                                             \
+    if (var == 0)                                                                                      \
+      return prec == 64 ? sweep_f64_##N_##_v0_b(row, mode, d, ninst, lx, others, s)                    \
+                        : sweep_f32_##N_##_v0_b(row, mode, d, ninst, lx, others, s);                   \
+    return prec == 64 ? sweep_f64_##N_##_v1_b(row, mode, d, ninst, lx, others, s)                      \
+                      : sweep_f32_##N_##_v1_b(row, mode, d, ninst, lx, others, s);
   switch (N) {
-    GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(64) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
+    GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
+    GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
+    default: *handled = false; return hipErrorInvalidValue;
+  }
+#undef GPUE_CASE
+}
+
+int sweep_blocks(bool row, int N, int var, int lines_or_xdim, int others) {
+#define GPUE_CASE(N_)                                                                              \
+  case N_:                                                                                         \
+    if (var == 0) return row ? row_blocks<N_, 0>(lines_or_xdim) : col_blocks<N_, 0>(lines_or_xdim, others); \
+    return row ? row_blocks<N_, 1>(lines_or_xdim) : col_blocks<N_, 1>(lines_or_xdim, others);
+  switch (N) {
+    GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
     GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
     default: return 0;
   }

@@ -1,0 +1,3 @@
+// generated: sweep kernels for N=1024, variant 1, double
+#include "sweep_launch.h"
+GPUE_SWEEP_TU(double, 1024, 1, sweep_f64_1024_v1)

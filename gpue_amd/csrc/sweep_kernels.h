@@ -4,25 +4,29 @@
 
 namespace gpue {
 
-template <int N>
+// Workgroup shapes.  Row sweeps: LPW lines of TPL lanes.  Strided sweeps: TC
+// adjacent x-lines (lanes vary fastest in x for coalescing) of TPL lanes.
+template <int N, int V>
 struct RowCfg {
-  static constexpr int TPL = Plan<N>::TPL;
-  static constexpr int WG = TPL >= 128 ? TPL : 128;
+  static constexpr int TPL = Plan<N, V>::TPL;
+  static constexpr int WGr = V == 0 ? 128 : 64;
+  static constexpr int WG = TPL >= WGr ? TPL : WGr;
   static constexpr int LPW = WG / TPL;
 };
 
-template <int N>
+template <int N, int V>
 struct ColCfg {
-  static constexpr int TPL = Plan<N>::TPL;
-  static constexpr int TCr = 256 / TPL;
+  static constexpr int TPL = Plan<N, V>::TPL;
+  static constexpr int TCr = (V == 0 ? 256 : 128) / TPL;
   static constexpr int TC = TCr > 64 ? 64 : (TCr < 1 ? 1 : TCr);
   static constexpr int WG = TPL * TC;
 };
 
+// scalarDiv_wfcNorm (src/kernels.cu:479-486) divides by sqrt(sum * dV); the
+// fused form multiplies by its reciprocal (within 1 ulp of the quotient).
 template <typename T>
-__device__ __forceinline__ cx<T> div_norm(cx<T> v, double nrm) {
-  // scalarDiv_wfcNorm (src/kernels.cu:479-486): psi / sqrt(sum * dV)
-  return mk((T)((double)v.x / nrm), (T)((double)v.y / nrm));
+__device__ __forceinline__ cx<T> scale_norm(cx<T> v, T inv) {
+  return mk(v.x * inv, v.y * inv);
 }
 
 template <typename T>
@@ -33,83 +37,83 @@ __device__ __forceinline__ double abs2d(cx<T> v) {
 // ---------------------------------------------------------------------------
 // Row sweep: lines are contiguous x-rows, line = y + yDim*z.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(RowCfg<N>::WG) row_sweep(SweepArgs a) {
-  using P = Plan<N>;
-  constexpr int TPL = P::TPL, E = P::E, LPW = RowCfg<N>::LPW;
+template <typename T, int N, int V, int MODE>
+__device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) {
+  using P = Plan<N, V>;
+  constexpr int TPL = P::TPL, E = P::E, LPW = RowCfg<N, V>::LPW;
   constexpr int FOOT = LdsRow::footprint(N);
   __shared__ cx<T> lds[P::NP > 1 ? LPW * FOOT : 1];
   __shared__ double sh[64];
   const Phys& p = a.ph;
   const unsigned flags = a.flags;
   const int tid = threadIdx.x % TPL, lw = threadIdx.x / TPL;
-  const int line = blockIdx.x * LPW + lw;
+  const int line = bid * LPW + lw;
   const bool active = line < a.lines;
   const int yi = active ? line % p.yDim : 0, zi = active ? line / p.yDim : 0;
   const cx<T>* __restrict__ in = (const cx<T>*)a.in + (long)(active ? line : 0) * N;
   cx<T>* __restrict__ out = (cx<T>*)a.out + (long)(active ? line : 0) * N;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
   const LdsRow la{lw * FOOT};
-  constexpr int RF0 = Seq<N, ORD_FWD>::first(), RFL = Seq<N, ORD_FWD>::last();
+  constexpr int RF0 = Seq<N, V, ORD_FWD>::first(), RFL = Seq<N, V, ORD_FWD>::last();
   cx<T> v[E];
   double acc = 0.0;
 
   if constexpr (MODE == RM_INV) {
     if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, RFL>(tid, r)];
+      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(tid, r)];
     }
-    fft_regs<T, N, ORD_REV, +1>(v, tid, tw, lds, la, active);
+    fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
     if (flags & F_HALFV_POST) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = op_halfV(p, v[r], bpos<N, RF0>(tid, r), yi, zi);
+      for (int r = 0; r < E; ++r) v[r] = op_halfV(p, v[r], bpos<N, TPL, RF0>(tid, r), yi, zi);
     }
     if (flags & F_GAUGE_POST) {
-      fft_regs<T, N, ORD_FWD, -1>(v, tid, tw, lds, la, active);
+      fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = op_A(p, v[r], 0, bpos<N, RFL>(tid, r), yi, zi, a.invN);
-      fft_regs<T, N, ORD_REV, +1>(v, tid, tw, lds, la, active);
+      for (int r = 0; r < E; ++r) v[r] = op_A(p, v[r], 0, bpos<N, TPL, RFL>(tid, r), yi, zi, a.invN);
+      fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
     }
   } else {
     if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, RF0>(tid, r)];
+      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RF0>(tid, r)];
     }
     if constexpr (MODE == RM_FWD || MODE == RM_FULL) {
       if (flags & F_SCALE_IN) {
-        const double nrm = *a.norm_in;
+        const T inv = (T)(1.0 / *a.norm_in);
 #pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = div_norm(v[r], nrm);
+        for (int r = 0; r < E; ++r) v[r] = scale_norm(v[r], inv);
       }
       if (flags & F_HALFV_PRE) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = op_halfV(p, v[r], bpos<N, RF0>(tid, r), yi, zi);
+        for (int r = 0; r < E; ++r) v[r] = op_halfV(p, v[r], bpos<N, TPL, RF0>(tid, r), yi, zi);
       }
     }
-    fft_regs<T, N, ORD_FWD, -1>(v, tid, tw, lds, la, active);
+    fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
     if constexpr (MODE == RM_GAUGE) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = op_A(p, v[r], 0, bpos<N, RFL>(tid, r), yi, zi, a.invN);
-      fft_regs<T, N, ORD_REV, +1>(v, tid, tw, lds, la, active);
+      for (int r = 0; r < E; ++r) v[r] = op_A(p, v[r], 0, bpos<N, TPL, RFL>(tid, r), yi, zi, a.invN);
+      fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
     } else if constexpr (MODE == RM_FULL) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = op_K(p, v[r], bpos<N, RFL>(tid, r), yi, zi, a.invN);
-      fft_regs<T, N, ORD_REV, +1>(v, tid, tw, lds, la, active);
+      for (int r = 0; r < E; ++r) v[r] = op_K(p, v[r], bpos<N, TPL, RFL>(tid, r), yi, zi, a.invN);
+      fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
       if (flags & F_HALFV_POST) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = op_halfV(p, v[r], bpos<N, RF0>(tid, r), yi, zi);
+        for (int r = 0; r < E; ++r) v[r] = op_halfV(p, v[r], bpos<N, TPL, RF0>(tid, r), yi, zi);
       }
       if (flags & F_GAUGE_POST) {
-        fft_regs<T, N, ORD_FWD, -1>(v, tid, tw, lds, la, active);
+        fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
 #pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = op_A(p, v[r], 0, bpos<N, RFL>(tid, r), yi, zi, a.invN);
-        fft_regs<T, N, ORD_REV, +1>(v, tid, tw, lds, la, active);
+        for (int r = 0; r < E; ++r) v[r] = op_A(p, v[r], 0, bpos<N, TPL, RFL>(tid, r), yi, zi, a.invN);
+        fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
       }
     } else if constexpr (MODE == RM_KSUM) {
       if (active) {
 #pragma unroll
         for (int r = 0; r < E; ++r) {
-          const int xi = bpos<N, RFL>(tid, r);
+          const int xi = bpos<N, TPL, RFL>(tid, r);
           acc += K_val(p, xi, yi, zi, gid_of(p, xi, yi, zi)) * abs2d(v[r]);
         }
       }
@@ -120,20 +124,20 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row_sweep(SweepArgs a) {
   if constexpr (MODE == RM_FWD) {
     if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) out[bpos<N, RFL>(tid, r)] = v[r];
+      for (int r = 0; r < E; ++r) out[bpos<N, TPL, RFL>(tid, r)] = v[r];
     }
   } else if constexpr (MODE != RM_KSUM) {
     if (active) {
 #pragma unroll
       for (int r = 0; r < E; ++r) {
-        out[bpos<N, RF0>(tid, r)] = v[r];
+        out[bpos<N, TPL, RF0>(tid, r)] = v[r];
         if (flags & F_NORM) acc += abs2d(v[r]);
       }
     }
   }
   if (MODE == RM_KSUM || (flags & F_NORM)) {
     double pv[1] = {acc};
-    block_partial<1>(pv, a.partials, sh);
+    block_partial<1>(pv, a.partials, sh, bid, nblk);
   }
 }
 
@@ -141,17 +145,17 @@ __global__ void __launch_bounds__(RowCfg<N>::WG) row_sweep(SweepArgs a) {
 // Strided sweep: lines along y (axis 1, stride xDim) or z (axis 2, stride
 // xDim*yDim); a workgroup owns TC adjacent x-lines, lanes vary fastest in x.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(ColCfg<N>::WG) col_sweep(SweepArgs a) {
-  using P = Plan<N>;
-  constexpr int TC = ColCfg<N>::TC, E = P::E;
+template <typename T, int N, int V, int MODE>
+__device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) {
+  using P = Plan<N, V>;
+  constexpr int TC = ColCfg<N, V>::TC, E = P::E, TPL = P::TPL;
   __shared__ cx<T> lds[P::NP > 1 ? TC * N : 1];
   __shared__ double sh[64];
   const Phys& p = a.ph;
   const unsigned flags = a.flags;
   const int c = threadIdx.x % TC, tid = threadIdx.x / TC;
   const int tiles = (p.xDim + TC - 1) / TC;
-  const int xt = blockIdx.x % tiles, o = blockIdx.x / tiles;
+  const int xt = bid % tiles, o = bid / tiles;
   const int xi = xt * TC + c;
   const bool active = xi < p.xDim;
   const long es = (a.axis == 1) ? (long)p.xDim : (long)p.xDim * p.yDim;
@@ -161,7 +165,7 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col_sweep(SweepArgs a) {
   cx<T>* __restrict__ out = (cx<T>*)a.out + base;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
   const LdsCol la{c, TC};
-  constexpr int RF0 = Seq<N, ORD_FWD>::first(), RFL = Seq<N, ORD_FWD>::last();
+  constexpr int RF0 = Seq<N, V, ORD_FWD>::first(), RFL = Seq<N, V, ORD_FWD>::last();
   cx<T> v[E];
   double acc = 0.0;
   // coordinates of element e along the axis
@@ -172,30 +176,30 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col_sweep(SweepArgs a) {
   if constexpr (MODE == CM_INV) {
     if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, RFL>(tid, r) * es];
+      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(tid, r) * es];
     }
-    fft_regs<T, N, ORD_REV, +1>(v, tid, tw, lds, la, active);
+    fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
     if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) out[bpos<N, RF0>(tid, r) * es] = v[r];
+      for (int r = 0; r < E; ++r) out[bpos<N, TPL, RF0>(tid, r) * es] = v[r];
     }
   } else {
     if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, RF0>(tid, r) * es];
+      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RF0>(tid, r) * es];
     }
-    fft_regs<T, N, ORD_FWD, -1>(v, tid, tw, lds, la, active);
+    fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
     if constexpr (MODE == CM_FWD) {
       if (active) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) out[bpos<N, RFL>(tid, r) * es] = v[r];
+        for (int r = 0; r < E; ++r) out[bpos<N, TPL, RFL>(tid, r) * es] = v[r];
       }
     } else if constexpr (MODE == CM_KSUM) {
       if (active) {
 #pragma unroll
         for (int r = 0; r < E; ++r) {
           int yi, zi;
-          yz(bpos<N, RFL>(tid, r), yi, zi);
+          yz(bpos<N, TPL, RFL>(tid, r), yi, zi);
           acc += K_val(p, xi, yi, zi, gid_of(p, xi, yi, zi)) * abs2d(v[r]);
         }
       }
@@ -203,15 +207,15 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col_sweep(SweepArgs a) {
 #pragma unroll
       for (int r = 0; r < E; ++r) {
         int yi, zi;
-        yz(bpos<N, RFL>(tid, r), yi, zi);
+        yz(bpos<N, TPL, RFL>(tid, r), yi, zi);
         if constexpr (MODE == CM_K) v[r] = op_K(p, v[r], xi, yi, zi, a.invN);
         else v[r] = op_A(p, v[r], a.axis, xi, yi, zi, a.invN);
       }
-      fft_regs<T, N, ORD_REV, +1>(v, tid, tw, lds, la, active);
+      fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
       if (active) {
 #pragma unroll
         for (int r = 0; r < E; ++r) {
-          out[bpos<N, RF0>(tid, r) * es] = v[r];
+          out[bpos<N, TPL, RF0>(tid, r) * es] = v[r];
           if (flags & F_NORM) acc += abs2d(v[r]);
         }
       }
@@ -219,8 +223,33 @@ __global__ void __launch_bounds__(ColCfg<N>::WG) col_sweep(SweepArgs a) {
   }
   if (MODE == CM_KSUM || (flags & F_NORM)) {
     double pv[1] = {acc};
-    block_partial<1>(pv, a.partials, sh);
+    block_partial<1>(pv, a.partials, sh, bid, nblk);
   }
+}
+
+// One problem per launch (arguments by value) ...
+template <typename T, int N, int V, int MODE>
+__global__ void __launch_bounds__((RowCfg<N, V>::WG)) row_sweep(SweepArgs a) {
+  row_body<T, N, V, MODE>(a, blockIdx.x, gridDim.x);
+}
+template <typename T, int N, int V, int MODE>
+__global__ void __launch_bounds__((ColCfg<N, V>::WG)) col_sweep(SweepArgs a) {
+  col_body<T, N, V, MODE>(a, blockIdx.x, gridDim.x);
+}
+
+// ... or many problems of one shape per launch: blocks [i*bpi, (i+1)*bpi) work
+// on instance i, whose arguments are read from device memory.
+template <typename T, int N, int V, int MODE>
+__global__ void __launch_bounds__((RowCfg<N, V>::WG)) row_sweep_b(const SweepArgs* __restrict__ args, int bpi) {
+  const SweepArgs& a = args[blockIdx.x / bpi];
+  if (!a.active) return;
+  row_body<T, N, V, MODE>(a, blockIdx.x % bpi, bpi);
+}
+template <typename T, int N, int V, int MODE>
+__global__ void __launch_bounds__((ColCfg<N, V>::WG)) col_sweep_b(const SweepArgs* __restrict__ args, int bpi) {
+  const SweepArgs& a = args[blockIdx.x / bpi];
+  if (!a.active) return;
+  col_body<T, N, V, MODE>(a, blockIdx.x % bpi, bpi);
 }
 
 }  // namespace gpue

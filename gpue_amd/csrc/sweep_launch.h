@@ -4,35 +4,64 @@
 
 namespace gpue {
 
-template <int N>
-inline int row_blocks(int lines) { return (lines + RowCfg<N>::LPW - 1) / RowCfg<N>::LPW; }
-template <int N>
-inline int col_blocks(int xDim, int others) { return ((xDim + ColCfg<N>::TC - 1) / ColCfg<N>::TC) * others; }
+template <int N, int V>
+inline int row_blocks(int lines) { return (lines + RowCfg<N, V>::LPW - 1) / RowCfg<N, V>::LPW; }
+template <int N, int V>
+inline int col_blocks(int xDim, int others) { return ((xDim + ColCfg<N, V>::TC - 1) / ColCfg<N, V>::TC) * others; }
 
-template <typename T, int N>
+template <typename T, int N, int V>
 hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
-  dim3 grid(row_blocks<N>(a.lines)), block(RowCfg<N>::WG);
+  dim3 grid(row_blocks<N, V>(a.lines)), block(RowCfg<N, V>::WG);
   switch (mode) {
-    case RM_FWD: hipLaunchKernelGGL((row_sweep<T, N, RM_FWD>), grid, block, 0, s, a); break;
-    case RM_INV: hipLaunchKernelGGL((row_sweep<T, N, RM_INV>), grid, block, 0, s, a); break;
-    case RM_GAUGE: hipLaunchKernelGGL((row_sweep<T, N, RM_GAUGE>), grid, block, 0, s, a); break;
-    case RM_FULL: hipLaunchKernelGGL((row_sweep<T, N, RM_FULL>), grid, block, 0, s, a); break;
-    case RM_KSUM: hipLaunchKernelGGL((row_sweep<T, N, RM_KSUM>), grid, block, 0, s, a); break;
+    case RM_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FWD>), grid, block, 0, s, a); break;
+    case RM_INV: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV>), grid, block, 0, s, a); break;
+    case RM_GAUGE: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE>), grid, block, 0, s, a); break;
+    case RM_FULL: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FULL>), grid, block, 0, s, a); break;
+    case RM_KSUM: hipLaunchKernelGGL((row_sweep<T, N, V, RM_KSUM>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <typename T, int N>
+template <typename T, int N, int V>
 hipError_t launch_col_t(int mode, const SweepArgs& a, hipStream_t s) {
   const int others = a.axis == 1 ? a.ph.zDim : a.ph.yDim;
-  dim3 grid(col_blocks<N>(a.ph.xDim, others)), block(ColCfg<N>::WG);
+  dim3 grid(col_blocks<N, V>(a.ph.xDim, others)), block(ColCfg<N, V>::WG);
   switch (mode) {
-    case CM_FWD: hipLaunchKernelGGL((col_sweep<T, N, CM_FWD>), grid, block, 0, s, a); break;
-    case CM_INV: hipLaunchKernelGGL((col_sweep<T, N, CM_INV>), grid, block, 0, s, a); break;
-    case CM_K: hipLaunchKernelGGL((col_sweep<T, N, CM_K>), grid, block, 0, s, a); break;
-    case CM_GAUGE: hipLaunchKernelGGL((col_sweep<T, N, CM_GAUGE>), grid, block, 0, s, a); break;
-    case CM_KSUM: hipLaunchKernelGGL((col_sweep<T, N, CM_KSUM>), grid, block, 0, s, a); break;
+    case CM_FWD: hipLaunchKernelGGL((col_sweep<T, N, V, CM_FWD>), grid, block, 0, s, a); break;
+    case CM_INV: hipLaunchKernelGGL((col_sweep<T, N, V, CM_INV>), grid, block, 0, s, a); break;
+    case CM_K: hipLaunchKernelGGL((col_sweep<T, N, V, CM_K>), grid, block, 0, s, a); break;
+    case CM_GAUGE: hipLaunchKernelGGL((col_sweep<T, N, V, CM_GAUGE>), grid, block, 0, s, a); break;
+    case CM_KSUM: hipLaunchKernelGGL((col_sweep<T, N, V, CM_KSUM>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Batched: `ninst` problems of one shape, arguments in device memory.
+template <typename T, int N, int V>
+hipError_t launch_row_b_t(int mode, const SweepArgs* d, int ninst, int lines, hipStream_t s) {
+  const int bpi = row_blocks<N, V>(lines);
+  dim3 grid(bpi * ninst), block(RowCfg<N, V>::WG);
+  switch (mode) {
+    case RM_FWD: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_FWD>), grid, block, 0, s, d, bpi); break;
+    case RM_INV: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_INV>), grid, block, 0, s, d, bpi); break;
+    case RM_GAUGE: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_GAUGE>), grid, block, 0, s, d, bpi); break;
+    case RM_FULL: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_FULL>), grid, block, 0, s, d, bpi); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T, int N, int V>
+hipError_t launch_col_b_t(int mode, const SweepArgs* d, int ninst, int xDim, int others, hipStream_t s) {
+  const int bpi = col_blocks<N, V>(xDim, others);
+  dim3 grid(bpi * ninst), block(ColCfg<N, V>::WG);
+  switch (mode) {
+    case CM_FWD: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_FWD>), grid, block, 0, s, d, bpi); break;
+    case CM_INV: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_INV>), grid, block, 0, s, d, bpi); break;
+    case CM_K: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_K>), grid, block, 0, s, d, bpi); break;
+    case CM_GAUGE: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_GAUGE>), grid, block, 0, s, d, bpi); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -40,10 +69,15 @@ hipError_t launch_col_t(int mode, const SweepArgs& a, hipStream_t s) {
 
 }  // namespace gpue
 
-// One translation unit per (precision, N) keeps the build parallel.
-#define GPUE_SWEEP_TU(T_, N_, NAME_)                                                              \
+// One translation unit per (precision, N, variant) keeps the build parallel.
+#define GPUE_SWEEP_TU(T_, N_, V_, NAME_)                                                          \
   namespace gpue {                                                                                \
   hipError_t NAME_(bool row, int mode, const SweepArgs& a, hipStream_t s) {                       \
-    return row ? launch_row_t<T_, N_>(mode, a, s) : launch_col_t<T_, N_>(mode, a, s);             \
+    return row ? launch_row_t<T_, N_, V_>(mode, a, s) : launch_col_t<T_, N_, V_>(mode, a, s);     \
+  }                                                                                               \
+  hipError_t NAME_##_b(bool row, int mode, const SweepArgs* d, int ninst, int lx, int others,     \
+                       hipStream_t s) {                                                           \
+    return row ? launch_row_b_t<T_, N_, V_>(mode, d, ninst, lx, s)                                \
+               : launch_col_b_t<T_, N_, V_>(mode, d, ninst, lx, others, s);                       \
   }                                                                                               \
   }

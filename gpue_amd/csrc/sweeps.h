@@ -61,10 +61,13 @@ struct Phys {
   const double *xg, *yg, *zg, *pxg, *pyg, *pzg;
   const double *vs[3];   // V_SEP: V = vs[0][x] + vs[1][y] + vs[2][z]
   const double *ks[3];   // K_SEP: K = ks[0][kx] + ks[1][ky] + ks[2][kz]
+  const double *kf[3];   // K_SEP: per-axis factors of the K operator for this gstate,
+                         // interleaved (re, im): exp(-K_a Dt/hbar) or exp(-i K_a Dt/hbar)
   const double *Vt, *Kt, *At[3];
 };
 
 struct SweepArgs {
+  int active;         // batched launches: 0 skips this instance
   const void* in;
   void* out;
   const void* tw;
@@ -109,16 +112,31 @@ __device__ __forceinline__ long gid_of(const Phys& p, int xi, int yi, int zi) {
   return (long)xi + (long)p.xDim * ((long)yi + (long)p.yDim * (long)zi);
 }
 
+// Transcendentals in the working precision: FP32 sweeps use the FP32 library
+// functions, FP64 sweeps the FP64 ones.
+template <typename T>
+struct WMath;
+template <>
+struct WMath<double> {
+  static __device__ __forceinline__ double ex(double a) { return exp(a); }
+  static __device__ __forceinline__ void sc(double a, double* s, double* c) { sincos(a, s, c); }
+};
+template <>
+struct WMath<float> {
+  static __device__ __forceinline__ float ex(float a) { return expf(a); }
+  static __device__ __forceinline__ void sc(float a, float* s, float* c) { sincosf(a, s, c); }
+};
+
 // psi * exp(-arg) (imaginary time) or psi * exp(-i arg) (real time), times s
 template <typename T>
 __device__ __forceinline__ cx<T> apply_phase(cx<T> v, double arg, int gstate, double s) {
   if (gstate == 0) {
-    T f = (T)(exp(-arg) * s);
+    T f = WMath<T>::ex((T)-arg) * (T)s;
     return cscale(v, f);
   }
-  double sn, cs;
-  sincos(-arg, &sn, &cs);
-  cx<T> w = mk((T)(cs * s), (T)(sn * s));
+  T sn, cs;
+  WMath<T>::sc((T)-arg, &sn, &cs);
+  cx<T> w = mk((T)(cs * (T)s), (T)(sn * (T)s));
   return cmul(w, v);
 }
 
@@ -136,9 +154,20 @@ __device__ __forceinline__ cx<T> op_halfV(const Phys& p, cx<T> v, int xi, int yi
   return apply_phase(v, arg, p.gstate, 1.0);
 }
 
-// K step: GK/EK (src/operators.cu:1082,1102) with both 1/sqrt(N) of src/evolution.cu:761,777
+// K step: GK/EK (src/operators.cu:1082,1102) with both 1/sqrt(N) of src/evolution.cu:761,777.
+// For the separable K the operator is the product of per-axis factors (no
+// transcendental per point).
 template <typename T>
 __device__ __forceinline__ cx<T> op_K(const Phys& p, cx<T> v, int xi, int yi, int zi, double invN) {
+  if (p.k_kind == K_SEP) {
+    const double* fx = p.kf[0] + 2 * xi;
+    const double* fy = p.kf[1] + 2 * yi;
+    const double* fz = p.kf[2] + 2 * zi;
+    if (p.gstate == 0) return cscale(v, (T)(fx[0] * fy[0] * fz[0] * invN));
+    const double ar = fx[0] * fy[0] - fx[1] * fy[1], ai = fx[0] * fy[1] + fx[1] * fy[0];
+    const double br = (ar * fz[0] - ai * fz[1]) * invN, bi = (ar * fz[1] + ai * fz[0]) * invN;
+    return cmul(mk((T)br, (T)bi), v);
+  }
   long gid = gid_of(p, xi, yi, zi);
   return apply_phase(v, K_val(p, xi, yi, zi, gid) * p.cK, p.gstate, invN);
 }
@@ -184,11 +213,11 @@ __device__ __forceinline__ void block_sum(double (&v)[NA], double* sh /* >= NA *
   }
 }
 
+// Block b of nb (of one problem) stores its NA partial sums.
 template <int NA>
-__device__ __forceinline__ void block_partial(double (&v)[NA], double* partials, double* sh) {
+__device__ __forceinline__ void block_partial(double (&v)[NA], double* partials, double* sh, unsigned b,
+                                              unsigned nb) {
   block_sum<NA>(v, sh);
-  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-  const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int a = 0; a < NA; ++a) partials[(size_t)a * nb + b] = v[a];

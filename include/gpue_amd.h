@@ -122,6 +122,18 @@ gpue_status gpue_set_g(gpue_ctx *c, double g);
 /* Wait for all work on the ctx stream. */
 gpue_status gpue_sync(gpue_ctx *c);
 
+/* FFT kernel shape per axis (no reference counterpart: cuFFT plans choose their
+ * own).  gpue_autotune times every variant of every sweep of the current
+ * configuration out of place and keeps the fastest per axis; variants[3]
+ * receives the choice (0 = wide lines, 1 = narrow lines).  gpue_set_variants
+ * forces a choice. */
+gpue_status gpue_autotune(gpue_ctx *c, int reps, int *variants);
+gpue_status gpue_set_variants(gpue_ctx *c, const int *variants);
+/* Steady-state steps are captured as a HIP graph of `steps` consecutive steps
+ * (even, default 8) and replayed; 0 launches every sweep individually.  No
+ * reference counterpart (GPUE launches each kernel from the host loop). */
+gpue_status gpue_set_graph_steps(gpue_ctx *c, int steps);
+
 /* --- measurement hooks (no reference counterpart: GPUE prints clock() deltas,
  * src/evolution.cu:436-441) --- */
 /* When on, gpue_evolve records a HIP event pair around every sweep launch. */
@@ -143,6 +155,63 @@ typedef struct {          /* EqnNode_gpu (include/ds.h:71-80) */
 gpue_status gpue_eqn_parse(const char *expr, const char *const *names, const double *values, int n_names,
                            gpue_eqn_node *nodes, int max_nodes, int *n_nodes, int *time_dependent);
 double      gpue_eqn_eval(const gpue_eqn_node *nodes, int n_nodes, double x, double y, double z, double t);
+
+/* --- batched instances (no reference counterpart: GPUE solves one problem per
+ * process; its bin/run.sh starts several processes).  Instances of one shape
+ * share every sweep launch; two steps of all instances are captured as a HIP
+ * graph and replayed.  Contexts stay owned by the caller and must outlive the
+ * batch; results are read through the contexts (gpue_energy, gpue_download_wfc).
+ * Contexts with a time-dependent V or on different devices are rejected. --- */
+typedef struct gpue_batch gpue_batch;
+gpue_status gpue_batch_create(gpue_ctx *const *ctxs, int n, gpue_batch **out);
+/* Exclude (0) or include (1) instance `index` from the following steps. */
+gpue_status gpue_batch_set_active(gpue_batch *b, int index, int active);
+/* Advance every active instance by nsteps steps (gstate 0 imaginary, 1 real);
+ * each instance continues from its own step index (GPUE's gauge-order parity). */
+gpue_status gpue_batch_evolve(gpue_batch *b, int nsteps, int gstate);
+const char *gpue_batch_last_error(const gpue_batch *b);
+void        gpue_batch_destroy(gpue_batch *b);
+
+/* --- solve service (no reference counterpart: GPUE runs one problem per
+ * process, bin/run.sh starts several).  A worker thread admits queued requests
+ * in arrival order while a slot and the HBM budget allow, keeps every state
+ * resident on the device and advances each admitted request on its own stream
+ * in chunks of check_every steps, each followed by an energy evaluation.  A
+ * request's result does not depend on the other requests in flight. --- */
+typedef struct {
+    gpue_params params;      /* problem; params.device is overridden by the server's */
+    unsigned long long seed; /* psi0 = exp(-r^2/2)(1 + 0.01 u), u uniform from the seed */
+    int random_phase;        /* 1: psi0 also gets a uniformly random phase per point */
+    int check_every;         /* steps between energy checks (<=0: 100) */
+    int max_imag_steps;      /* imaginary-time step limit */
+    double energy_tol;       /* stop imaginary time when |E_k - E_k-1| < tol |E_k-1| */
+    int real_steps;          /* real-time steps after the ground state (0: none) */
+    double quench_factor;    /* g is multiplied by this before real time */
+    double *psi_out;         /* optional: 2*N doubles receiving the final psi
+                                (must stay valid until the result is polled) */
+} gpue_request;
+
+typedef struct {
+    unsigned long long id;
+    int status;              /* gpue_status of the request */
+    int imag_steps, real_steps;
+    double energy, mu;       /* at the last check */
+    double t_submit, t_start, t_done;   /* seconds on the server clock */
+} gpue_result;
+
+typedef struct gpue_server gpue_server;
+/* budget_bytes: device memory the admitted requests may hold together;
+ * max_slots: requests in flight at once. */
+gpue_status gpue_server_create(int device, double budget_bytes, int max_slots, gpue_server **out);
+/* Queue a request; GPUE_EBUDGET if it can never fit the budget. Thread-safe. */
+gpue_status gpue_server_submit(gpue_server *s, const gpue_request *r, unsigned long long id);
+/* Move up to max finished results to out; returns how many. Thread-safe. */
+int         gpue_server_poll(gpue_server *s, gpue_result *out, int max);
+/* Wait until every submitted request has finished (GPUE_EBUDGET on timeout). */
+gpue_status gpue_server_drain(gpue_server *s, double timeout_s);
+double      gpue_server_clock(const gpue_server *s);
+/* Finishes the requests in flight and queued, then stops the worker. */
+void        gpue_server_destroy(gpue_server *s);
 
 #ifdef __cplusplus
 }

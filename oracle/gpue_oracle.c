@@ -649,3 +649,26 @@ int or_vort_pos(const int *marker, int xDim, int *out, int max_out) {
             }
     return n;
 }
+
+/* src/tracker.cu:317-345 (lsFit) with the matrix of include/tracker.h:46-48:
+ * least-squares plane psi ~ a i + b j + c through the plaquette corners, the
+ * sub-grid core at (i, j) - R with a R_i + b R_j = -c solved as GPUE does.
+ * ijw: n triples (i, j, winding); out: n pairs (xd, yd). */
+void or_ls_fit(const int *ijw, int n, const double *wfc_il, int xDim, double *out) {
+    static const double lsq[3][4] = {{-0.5, 0.5, -0.5, 0.5}, {-0.5, -0.5, 0.5, 0.5}, {0.75, 0.25, 0.25, -0.25}};
+    const or_c *wfc = (const or_c *)wfc_il;
+    for (int v = 0; v < n; ++v) {
+        int x = ijw[3 * v], y = ijw[3 * v + 1];
+        or_c g[4] = {wfc[x * xDim + y], wfc[(x + 1) * xDim + y], wfc[x * xDim + (y + 1)], wfc[(x + 1) * xDim + (y + 1)]};
+        or_c res[3];
+        for (int jj = 0; jj < 3; ++jj) {
+            res[jj].x = lsq[jj][0] * g[0].x + lsq[jj][1] * g[1].x + lsq[jj][2] * g[2].x + lsq[jj][3] * g[3].x;
+            res[jj].y = lsq[jj][0] * g[0].y + lsq[jj][1] * g[1].y + lsq[jj][2] * g[2].y + lsq[jj][3] * g[3].y;
+        }
+        double det = 1.0 / (res[0].x * res[1].y - res[0].y * res[1].x);
+        double Rx = det * (res[1].y * res[2].x - res[0].y * res[2].y);
+        double Ry = det * (-res[1].x * res[2].x + res[0].x * res[2].y);
+        out[2 * v] = x - Rx;
+        out[2 * v + 1] = y - Ry;
+    }
+}

@@ -79,6 +79,7 @@ void    or_parsum_raw(or_c *wfc, int dimnum, int xDim, int yDim, int zDim,
 int     or_find_vortex(int *marker, const double *wfc_interleaved, double radius,
                        int xDim, const double *x);
 int     or_vort_pos(const int *marker, int xDim, int *out_ij_wind, int max_out);
+void    or_ls_fit(const int *ijw, int n, const double *wfc_interleaved, int xDim, double *out_xy);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,7 @@ def lib():
         L.or_find_vortex.restype = ctypes.c_int
         L.or_vort_pos.argtypes = [ip, ctypes.c_int, ip, ctypes.c_int]
         L.or_vort_pos.restype = ctypes.c_int
+        L.or_ls_fit.argtypes = [ip, ctypes.c_int, dp, ctypes.c_int, dp]
         _lib = L
     return _lib
 
@@ -132,6 +133,16 @@ def vort_pos(marker):
     out = np.zeros(3 * max(n, 1), dtype=np.int32)
     lib().or_vort_pos(_ip(marker), xDim, _ip(out), n)
     return out[: 3 * n].reshape(n, 3)
+
+
+def ls_fit(ijw, wfc2d):
+    """GPUE Tracker::lsFit: sub-grid (xd, yd) for each (i, j, winding) row of ijw."""
+    ijw = np.ascontiguousarray(ijw, dtype=np.int32)
+    wfc2d = np.ascontiguousarray(wfc2d, dtype=np.complex128)
+    n = ijw.shape[0]
+    out = np.zeros(2 * max(n, 1))
+    lib().or_ls_fit(_ip(ijw), n, _dp(wfc2d.view(np.float64)), wfc2d.shape[1], _dp(out))
+    return out[: 2 * n].reshape(n, 2)
 
 
 class Oracle:

@@ -161,3 +161,15 @@ def test_early_stop_rule():
     assert done < 5000 and done % 10 == 1
     assert abs(en[-1] - en[-2]) < 1e-6 * en[-2]
     assert abs(en[-2] - en[-3]) >= 1e-6 * en[-3]
+
+
+def test_ls_fit_linear_vortex_closed_form():
+    # a field that is exactly linear in the grid indices, psi = (i - i0) + 1j (j - j0),
+    # is fitted exactly by the least-squares plane: the core is (i0, j0)
+    n = 16
+    I, J = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    for (i0, j0) in [(5.3, 7.8), (10.01, 2.99), (8.5, 8.5)]:
+        psi = (I - i0) + 1j * (J - j0)
+        ijw = np.array([[int(i0), int(j0), 1]], dtype=np.int32)
+        xy = O.ls_fit(ijw, psi)
+        assert abs(xy[0, 0] - i0) < 1e-12 and abs(xy[0, 1] - j0) < 1e-12
