@@ -11,9 +11,11 @@ resident in HBM when the timed region starts.
 
 Also reported (extra): the 3D ground state on 256^3 (V = (x^2+y^2+4z^2)/2,
 g = 5000, dt = 5e-4) in FP64 and FP32-bulk/FP64-reductions, the FP32/FP64
-speedup and energy error, per-kernel HIP-event timings and the
-roofline of the dominant kernels, single-solve latency of the 256^2 ground
-state, and a CPU baseline (the FP64 oracle on the host cores, bounded sample).
+speedup and energy error, per-kernel costs (each distinct sweep replayed back
+to back from a graph, HIP events on the context stream) and the roofline of
+the dominant kernel, single-solve latency of the 256^2 ground state, 64
+batched uneven FP32 solves, a Poisson serving stream, and a CPU baseline (the
+FP64 oracle on the host cores, bounded sample).
 """
 import argparse
 import json
@@ -69,22 +71,46 @@ def run_schedule(ctx, imag, real):
         ctx.evolve(real, 1)
 
 
-def kernel_table(timing, bytes_per_launch):
+RM = {"row_fwd": 0, "row_inv": 1, "row_gauge": 2, "row_full": 3}
+CM = {"fwd": 0, "inv": 1, "k": 2, "gauge": 3}
+
+
+def kernel_symbol(name, prec, n_axis, variant):
+    """The template instance rocprofv3 reports for a sweep name."""
+    t = "double" if prec == 64 else "float"
+    base = name.split("+")[0]
+    if base.startswith("row"):
+        return f"gpue::row_sweep<{t}, {n_axis[0]}, {variant[0]}, {RM[base]}>"
+    axis = 1 if base.startswith("coly") else 2
+    return f"gpue::col_sweep<{t}, {n_axis[axis]}, {variant[axis]}, {CM[base.split('_', 1)[1]]}>"
+
+
+def kernel_table(ctx, prec, variants, steps_by_gstate, reps=50):
+    """Per-kernel cost from gpue_sweep_profile: each distinct sweep replayed back
+    to back from a graph and timed with HIP events on the context stream."""
+    S = 16 if prec == 64 else 8
+    bytes_per_launch = 2 * S * ctx.N
+    dims = [ctx.params.xDim, ctx.params.yDim, ctx.params.zDim]
     rows = []
-    for name, (n, ms) in timing.items():
-        avg_us = ms / n * 1e3
-        rows.append(dict(kernel=name, launches=n, total_ms=ms, avg_us=avg_us,
-                         gbs=bytes_per_launch / (avg_us * 1e-6) / 1e9))
-    rows.sort(key=lambda r: -r["total_ms"])
-    return rows
+    for gstate, nsteps in steps_by_gstate.items():
+        if nsteps <= 0:
+            continue
+        for k in ctx.sweep_profile(gstate, reps):
+            rows.append(dict(kernel=k["name"], gstate=gstate, symbol=kernel_symbol(k["name"], prec, dims, variants),
+                             us_per_launch=round(k["us"], 3), launches_per_step=k["per_step"],
+                             job_ms=round(k["us"] * k["per_step"] * nsteps / 1e3, 3),
+                             GBps=round(bytes_per_launch / (k["us"] * 1e-6) / 1e9, 1)))
+    rows.sort(key=lambda r: -r["job_ms"])
+    return rows, bytes_per_launch
 
 
 def roofline(rows, bytes_per_launch):
     top = rows[0]
-    achieved = bytes_per_launch / (top["avg_us"] * 1e-6) / 1e9
-    return {"bound": "hbm", "kernel": top["kernel"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(top["avg_us"], 3)}
+    achieved = bytes_per_launch / (top["us_per_launch"] * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": top["kernel"],
+            "symbol": top["symbol"], "bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": top["us_per_launch"]}
 
 
 def cpu_baseline(budget_s):
@@ -168,13 +194,13 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=3000, tol=1e-8):
             "steps_to_converge": {"min": int(min(steps)), "max": int(max(steps))}}
 
 
-def serving_stream(G, n_req=48, rate=200.0, seed=5, budget=250e9):
+def serving_stream(G, n_req=48, rate=200.0, seed=5, budget=250e9, slots=8):
     """Poisson arrivals (mean rate req/s) of 2D ground states (128^2-1024^2) and
     3D quench runs (64^3-256^3: imaginary-time preparation, g quenched by
     U[0.5,2], 1000 real-time steps), each in FP64 or FP32-bulk, admitted by the
     HBM budget; reports throughput and p50/p99 latency (submit to result)."""
     rng = np.random.default_rng(seed)
-    sv = G.Server(budget_bytes=budget, max_slots=32)
+    sv = G.Server(budget_bytes=budget, max_slots=slots)
     reqs = []
     for k in range(n_req):
         prec = 64 if rng.uniform() < 0.5 else 32
@@ -216,7 +242,7 @@ def serving_stream(G, n_req=48, rate=200.0, seed=5, budget=250e9):
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
                            "p99": round(float(np.percentile(lat, 99)), 2),
                            "max": round(float(lat.max()), 2)},
-            "budget_GB": budget / 1e9, "max_slots": 32}
+            "budget_GB": budget / 1e9, "max_slots": slots}
 
 
 def main():
@@ -227,8 +253,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the 256^3 / latency extras")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=["512", "256c", "batch", "serve"], default=None,
-                    help="profiling: run one workload only")
+    ap.add_argument("--only", choices=["512", "256c", "lat", "batch", "serve", "copy"], default=None,
+                    help="profiling: run one workload only (copy: a plain 256 MiB device copy, the "
+                         "calibration kernel for the HBM byte counters)")
+    ap.add_argument("--steps3", type=int, default=200, help="timed 256^3 steps per precision")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; every solve runs on a single MI355X "
@@ -245,6 +273,7 @@ def main():
     # ---------------- headline: 512^2 rotating condensate, FP64, gauge ----------------
     if args.only in (None, "512"):
         ctx, psi0 = make_ctx(G, 64, CFG2, phase=True, noise=0.0)
+        variants = ctx.autotune()
         run_schedule(ctx, W // 2, W - W // 2)
         ctx.upload(psi0)
         ctx.sync()
@@ -257,35 +286,39 @@ def main():
         bps_imag, P_imag = ctx.step_bytes(0)
         bps_real, P_real = ctx.step_bytes(1)
         bytes_job = imag * bps_imag + real * bps_real
-        # second pass with a HIP event pair around every sweep launch (ctx stream)
-        ctx.upload(psi0)
-        ctx.set_timing(True)
-        ctx.sync()
-        t1 = time.perf_counter()
-        run_schedule(ctx, imag, real)
-        ctx.sync()
-        el_ev = time.perf_counter() - t1
-        rows = kernel_table(ctx.timing(), 2 * 16 * 512 * 512)
-        ctx.set_timing(False)
+        rows, bpl = kernel_table(ctx, 64, variants, {0: imag, 1: real})
         ctx.close()
-        log(f"512^2 FP64 gauge: {sps:.1f} steps/s ({el / K * 1e6:.2f} us/step), E={e_final:.10g}")
+        log(f"512^2 FP64 gauge: {sps:.1f} steps/s ({el / K * 1e6:.2f} us/step), E={e_final:.10g}, variants {variants}")
         for r in rows:
-            log(f"   {r['kernel']:30s} n={r['launches']:6d} avg={r['avg_us']:8.2f} us  {r['gbs']:8.1f} GB/s")
-        roof = roofline(rows, 2 * 16 * 512 * 512)
+            log(f"   {r['kernel']:24s} g{r['gstate']} {r['us_per_launch']:8.2f} us x {r['launches_per_step']:.1f}/step "
+                f"{r['GBps']:8.1f} GB/s  {r['symbol']}")
+        roof = roofline(rows, bpl)
         extra["512sq_fp64_gauge"] = {
             "steps_per_s": round(sps, 1), "us_per_step": round(el / K * 1e6, 3),
             "effective_GBps": round(bytes_job / el / 1e9, 1), "sweeps_per_step_imag": P_imag,
             "sweeps_per_step_real": P_real, "energy_final": e_final, "mu_final": mu_final,
-            "events_pass_steps_per_s": round(K / el_ev, 1), "kernels": rows}
+            "variants": list(variants), "kernels": rows}
     else:
         sps, el, roof = float("nan"), float("nan"), None
 
     # ---------------- 256^3 FP64 vs FP32 ----------------
+    if args.only == "copy":
+        import torch
+
+        a = torch.ones(1 << 24, dtype=torch.complex128, device="cuda")   # 256 MiB
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy: 5 x 256 MiB device-to-device (268435456 bytes read, 268435456 written per launch)")
+        return
+
     if not args.no_extra and args.only in (None, "256c"):
-        K3 = 200
+        K3 = args.steps3
         res3 = {}
         for prec in (64, 32):
             ctx, psi0 = make_ctx(G, prec, CFG3, seed=1)
+            variants = ctx.autotune()
             ctx.evolve(10, 0)
             ctx.upload(psi0)
             ctx.sync()
@@ -295,27 +328,25 @@ def main():
             el3 = time.perf_counter() - t0
             e3, mu3 = ctx.energy()
             b3, P3 = ctx.step_bytes(0)
-            ctx.upload(psi0)
-            ctx.set_timing(True)
-            ctx.evolve(50, 0)
-            ctx.sync()
-            rows3 = kernel_table(ctx.timing(), 2 * (16 if prec == 64 else 8) * 256 ** 3)
-            ctx.set_timing(False)
+            rows3, bpl3 = kernel_table(ctx, prec, variants, {0: K3}, reps=10)
             ctx.close()
             res3[prec] = dict(steps_per_s=round(K3 / el3, 2), us_per_step=round(el3 / K3 * 1e6, 2),
                               effective_GBps=round(b3 * K3 / el3 / 1e9, 1),
                               frac_of_8TBps=round(b3 * K3 / el3 / 1e9 / HBM_PEAK_GBS, 4), sweeps_per_step=P3,
-                              energy=e3, mu=mu3, kernels=rows3,
-                              roofline=roofline(rows3, 2 * (16 if prec == 64 else 8) * 256 ** 3))
-            log(f"256^3 FP{prec}: {K3 / el3:.1f} steps/s, {b3 * K3 / el3 / 1e9:.0f} GB/s effective, E={e3:.12g}")
+                              energy=e3, mu=mu3, variants=list(variants), kernels=rows3,
+                              roofline=roofline(rows3, bpl3))
+            log(f"256^3 FP{prec}: {K3 / el3:.1f} steps/s, {b3 * K3 / el3 / 1e9:.0f} GB/s effective, E={e3:.12g}, "
+                f"variants {variants}")
             for r in rows3:
-                log(f"   {r['kernel']:30s} n={r['launches']:6d} avg={r['avg_us']:8.2f} us  {r['gbs']:8.1f} GB/s")
+                log(f"   {r['kernel']:24s} {r['us_per_launch']:8.2f} us x {r['launches_per_step']:.1f}/step "
+                    f"{r['GBps']:8.1f} GB/s  {r['symbol']}")
         extra["256cube_fp64"] = res3[64]
         extra["256cube_fp32"] = res3[32]
         extra["fp32_over_fp64_speedup_256cube"] = round(res3[32]["steps_per_s"] / res3[64]["steps_per_s"], 3)
         extra["fp32_rel_energy_error_256cube"] = abs(res3[32]["energy"] - res3[64]["energy"]) / abs(res3[64]["energy"])
         extra["fp32_rel_mu_error_256cube"] = abs(res3[32]["mu"] - res3[64]["mu"]) / abs(res3[64]["mu"])
 
+    if not args.no_extra and args.only in (None, "lat"):
         # single-solve latency of the 256^2 ground state (config 1: 2000 imaginary steps)
         lat = []
         for rep in range(7):

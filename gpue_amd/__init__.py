@@ -34,7 +34,7 @@ ABI_SYMBOLS = [
     "gpue_params_default", "gpue_create", "gpue_destroy", "gpue_last_error", "gpue_set_field",
     "gpue_set_field_eqn", "gpue_upload_wfc", "gpue_download_wfc", "gpue_evolve", "gpue_parsum",
     "gpue_energy", "gpue_vortex_marker_2d", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
-    "gpue_sync", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_eqn_parse", "gpue_eqn_eval",
+    "gpue_sync", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
     "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_evolve", "gpue_batch_last_error", "gpue_batch_destroy",
     "gpue_server_create", "gpue_server_submit", "gpue_server_poll", "gpue_server_drain", "gpue_server_clock",
     "gpue_server_destroy",
@@ -109,6 +109,8 @@ def lib():
         L.gpue_set_timing.argtypes = [vp, ctypes.c_int]
         L.gpue_timing_get.argtypes = [vp, ctypes.c_int, ctypes.POINTER(cp), ctypes.POINTER(ctypes.c_long), dp]
         L.gpue_step_bytes.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.gpue_sweep_profile.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, ctypes.POINTER(cp), dp, dp]
+        L.gpue_math_eval.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_long]
         L.gpue_eqn_parse.argtypes = [cp, ctypes.POINTER(cp), dp, ctypes.c_int, ctypes.POINTER(EqnNode),
                                      ctypes.c_int, ip, ip]
         L.gpue_eqn_eval.argtypes = [ctypes.POINTER(EqnNode), ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -158,6 +160,16 @@ def make_params(**kw):
     for k, v in base.items():
         setattr(p, k, v)
     return p
+
+
+def math_eval(fn, x, device=0):
+    """Device FP64 exp / sin / cos of the fused operators (fn "exp", "sin", "cos")."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    st = lib().gpue_math_eval(device, {"exp": 0, "sin": 1, "cos": 2}[fn], _dp(x), _dp(y), x.size)
+    if st != GPUE_OK:
+        raise GpueError(st, "gpue_math_eval failed")
+    return y
 
 
 def _names(mapping):
@@ -299,6 +311,19 @@ class Context:
 
     def set_graph_steps(self, steps):
         self._check(lib().gpue_set_graph_steps(self._h, steps), "set_graph_steps")
+
+    def sweep_profile(self, gstate, reps=50):
+        """Per-kernel cost of the step: [{name, us, per_step}] from back-to-back
+        graph replays of each distinct sweep (in place on a scratch copy of psi), HIP events on the
+        context stream."""
+        m = 32
+        names = (ctypes.c_char_p * m)()
+        us = np.zeros(m)
+        per = np.zeros(m)
+        n = ctypes.c_int()
+        self._check(lib().gpue_sweep_profile(self._h, gstate, reps, m, ctypes.byref(n), names, _dp(us), _dp(per)),
+                    "sweep_profile")
+        return [dict(name=names[k].decode(), us=float(us[k]), per_step=float(per[k])) for k in range(min(n.value, m))]
 
     def set_variants(self, variants):
         v = np.ascontiguousarray(variants, dtype=np.int32)

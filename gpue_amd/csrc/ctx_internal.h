@@ -76,6 +76,7 @@ struct gpue_ctx {
   std::vector<std::string> kind_names;
   std::vector<long> kind_n;
   std::vector<double> kind_ms;
+  std::vector<std::string> prof_names;   // names handed out by gpue_sweep_profile
 };
 
 

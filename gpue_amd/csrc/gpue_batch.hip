@@ -4,12 +4,15 @@
 // variants, gauge/no gauge) form a group.  For every sweep of the step one
 // batched kernel covers every instance of the group: blocks [i*bpi, (i+1)*bpi)
 // work on instance i, whose SweepArgs live in a device array.  GPUE's gauge
-// order alternates with the step parity (src/evolution.cu:837,842), so each
-// group has an even-step and an odd-step sequence; instances whose own step
-// counters differ in parity run in the other sequence of the same step (their
-// entries are inactive in the first).  Two consecutive steps of all groups are
-// captured once per gstate as a HIP graph and replayed.  Device memory is
-// allocated only in gpue_batch_create.
+// order alternates with the step parity (src/evolution.cu:837,842).  When the
+// odd-step sweep list is a prefix of the even-step one in kernel kind (the 2D
+// and rotation-about-z cases: [FWD][K][INV(+A_x)][A_y(+norm)] and
+// [FWD][K][INV][A_y][A_x+norm]) both parities share one launch per position,
+// with per-instance flags, and the extra position is inactive for odd
+// instances; otherwise each parity has its own sequence.  Every imaginary-time
+// sequence ends with one batched norm finalize.  Two consecutive steps of all
+// groups are captured once per gstate as a HIP graph and replayed.  Device
+// memory is allocated only in gpue_batch_create.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -21,24 +24,36 @@ using namespace gpue;
 
 namespace {
 
-struct BSeq {                       // one (gstate, schedule parity, phase) sequence of a group
-  std::vector<Sweep> sweeps;
-  std::vector<SweepArgs*> dargs;   // per sweep: device array [members]
-  std::vector<FinArgs*> dfin;      // per sweep: device array [members] or null
+struct BSeq {                       // one launch sequence of a group for (gstate, phase)
+  std::vector<Sweep> keys;          // kernel kind per position (flags come per instance)
+  int mask = 3;                     // step parities it serves (bit p: parity p)
+  std::vector<SweepArgs*> dargs;    // per position: device array [members]
   std::vector<std::vector<SweepArgs>> hargs;
-  std::vector<std::vector<FinArgs>> hfin;
+  FinArgs* dfin = nullptr;          // imaginary time: device array [members]
+  std::vector<FinArgs> hfin;
 };
 
 struct Group {
   std::vector<int> members;
-  gpue_ctx* proto;                 // first member: shape and variants
-  BSeq seq[2][2][2];                // [gstate][schedule parity][phase]
+  gpue_ctx* proto;                  // first member: shape and variants
+  std::vector<Sweep> sched[2][2];   // [gstate][parity]
+  std::vector<BSeq> seqs[2][2];     // [gstate][phase]
 };
 
 bool same_shape(const gpue_ctx* a, const gpue_ctx* b) {
   return a->prec == b->prec && a->p.dimnum == b->p.dimnum && a->dims[0] == b->dims[0] && a->dims[1] == b->dims[1] &&
          a->dims[2] == b->dims[2] && a->var[0] == b->var[0] && a->var[1] == b->var[1] && a->var[2] == b->var[2] &&
          a->p.corotating == b->p.corotating && a->az_zero == b->az_zero && a->p.device == b->p.device;
+}
+
+bool same_kind(const Sweep& x, const Sweep& y) { return x.row == y.row && x.axis == y.axis && x.mode == y.mode; }
+
+// the shorter list's kinds are a prefix of the longer list's
+bool mergeable(const std::vector<Sweep>& a, const std::vector<Sweep>& b) {
+  const size_t n = std::min(a.size(), b.size());
+  for (size_t k = 0; k < n; ++k)
+    if (!same_kind(a[k], b[k])) return false;
+  return true;
 }
 
 }  // namespace
@@ -69,25 +84,27 @@ static gpue_status bfail(gpue_batch* b, gpue_status s, const std::string& m) {
 
 // Fill the host argument arrays of every sequence from the members' current state.
 static void fill_args(gpue_batch* b) {
-  for (Group& g : b->groups) {
+  for (Group& g : b->groups)
     for (int gs = 0; gs < 2; ++gs)
-      for (int par = 0; par < 2; ++par)
-        for (int ph = 0; ph < 2; ++ph) {
-          BSeq& s = g.seq[gs][par][ph];
-          for (size_t k = 0; k < s.sweeps.size(); ++k) {
-            const Sweep& sw = s.sweeps[k];
-            for (size_t m = 0; m < g.members.size(); ++m) {
-              const int idx = g.members[m];
-              gpue_ctx* c = b->ctx[idx];
-              // the instance runs this sequence when its step parity at this phase is `par`
-              const int on = b->active[idx] && ((b->cls[idx] + ph) % 2 == par);
+      for (int ph = 0; ph < 2; ++ph)
+        for (BSeq& s : g.seqs[gs][ph])
+          for (size_t m = 0; m < g.members.size(); ++m) {
+            const int idx = g.members[m];
+            gpue_ctx* c = b->ctx[idx];
+            const int par = (b->cls[idx] + ph) % 2;   // the instance's step parity at this phase
+            const bool in = b->active[idx] && ((s.mask >> par) & 1);
+            const std::vector<Sweep>& sc = g.sched[gs][par];
+            const Phys phys = make_phys(c, gs, 0.0);
+            for (size_t k = 0; k < s.keys.size(); ++k) {
+              const bool on = in && k < sc.size();
+              const Sweep& sw = on ? sc[k] : s.keys[k];
               SweepArgs& a = s.hargs[k][m];
               std::memset(&a, 0, sizeof(a));
               a.active = on;
               a.in = c->psi;
               a.out = c->psi;
               a.tw = c->tw[c->dims[sw.axis]];
-              a.ph = make_phys(c, gs, 0.0);
+              a.ph = phys;
               a.lines = c->dims[1] * c->dims[2];
               a.axis = sw.axis;
               a.gauge_axis = sw.axis;
@@ -95,38 +112,36 @@ static void fill_args(gpue_batch* b) {
               a.invN = sw.invN;
               a.norm_in = c->results;
               a.partials = c->partials;
-              if (s.dfin[k]) {
-                FinArgs& f = s.hfin[k][m];
-                const int n = c->dims[sw.axis];
-                const int var = c->var[sw.axis];
-                f.active = on;
-                f.nb = sw.row ? sweep_blocks(true, n, var, c->dims[1] * c->dims[2], 1)
-                              : sweep_blocks(false, n, var, c->dims[0], sw.axis == 1 ? c->dims[2] : c->dims[1]);
-                f.na = 1;
-                f.mode = 1;
-                f.scale = c->d[0] * c->d[1] * c->d[2];
-                f.partials = c->partials;
-                f.out = c->results;
-              }
+            }
+            if (s.dfin) {
+              // the partials come from the instance's F_NORM sweep (its last)
+              const Sweep& last = sc.back();
+              const int n = c->dims[last.axis];
+              const int var = c->var[last.axis];
+              FinArgs& f = s.hfin[m];
+              std::memset(&f, 0, sizeof(f));
+              f.active = in;
+              f.nb = last.row ? sweep_blocks(true, n, var, c->dims[1] * c->dims[2], 1)
+                              : sweep_blocks(false, n, var, c->dims[0], last.axis == 1 ? c->dims[2] : c->dims[1]);
+              f.na = 1;
+              f.mode = 1;
+              f.scale = c->d[0] * c->d[1] * c->d[2];
+              f.partials = c->partials;
+              f.out = c->results;
             }
           }
-        }
-  }
 }
 
 static gpue_status upload_args(gpue_batch* b) {
   for (Group& g : b->groups)
     for (int gs = 0; gs < 2; ++gs)
-      for (int par = 0; par < 2; ++par)
-        for (int ph = 0; ph < 2; ++ph) {
-          BSeq& s = g.seq[gs][par][ph];
-          for (size_t k = 0; k < s.sweeps.size(); ++k) {
+      for (int ph = 0; ph < 2; ++ph)
+        for (BSeq& s : g.seqs[gs][ph]) {
+          for (size_t k = 0; k < s.keys.size(); ++k)
             BHIP(b, hipMemcpy(s.dargs[k], s.hargs[k].data(), s.hargs[k].size() * sizeof(SweepArgs),
                               hipMemcpyHostToDevice));
-            if (s.dfin[k])
-              BHIP(b, hipMemcpy(s.dfin[k], s.hfin[k].data(), s.hfin[k].size() * sizeof(FinArgs),
-                                hipMemcpyHostToDevice));
-          }
+          if (s.dfin)
+            BHIP(b, hipMemcpy(s.dfin, s.hfin.data(), s.hfin.size() * sizeof(FinArgs), hipMemcpyHostToDevice));
         }
   return GPUE_OK;
 }
@@ -135,10 +150,9 @@ static gpue_status upload_args(gpue_batch* b) {
 static gpue_status launch_phase(gpue_batch* b, int gs, int ph) {
   for (Group& g : b->groups) {
     gpue_ctx* c = g.proto;
-    for (int par = 0; par < 2; ++par) {
-      BSeq& s = g.seq[gs][par][ph];
-      for (size_t k = 0; k < s.sweeps.size(); ++k) {
-        const Sweep& sw = s.sweeps[k];
+    for (BSeq& s : g.seqs[gs][ph]) {
+      for (size_t k = 0; k < s.keys.size(); ++k) {
+        const Sweep& sw = s.keys[k];
         const int n = c->dims[sw.axis];
         const int lx = sw.row ? c->dims[1] * c->dims[2] : c->dims[0];
         const int others = sw.row ? 1 : (sw.axis == 1 ? c->dims[2] : c->dims[1]);
@@ -146,10 +160,10 @@ static gpue_status launch_phase(gpue_batch* b, int gs, int ph) {
         hipError_t e = launch_sweep_batched(c->prec, sw.row, n, c->var[sw.axis], sw.mode, s.dargs[k],
                                             (int)g.members.size(), lx, others, b->stream, &handled);
         if (!handled || e != hipSuccess) return bfail(b, GPUE_EHIP, "batched sweep launch");
-        if (s.dfin[k]) {
-          e = launch_finalize_batched(s.dfin[k], (int)g.members.size(), b->stream);
-          if (e != hipSuccess) return bfail(b, GPUE_EHIP, "batched finalize launch");
-        }
+      }
+      if (s.dfin) {
+        hipError_t e = launch_finalize_batched(s.dfin, (int)g.members.size(), b->stream);
+        if (e != hipSuccess) return bfail(b, GPUE_EHIP, "batched finalize launch");
       }
     }
   }
@@ -173,13 +187,11 @@ void gpue_batch_destroy(gpue_batch* b) {
   drop_batch_graphs(b);
   for (Group& g : b->groups)
     for (int gs = 0; gs < 2; ++gs)
-      for (int par = 0; par < 2; ++par)
-        for (int ph = 0; ph < 2; ++ph) {
-          BSeq& s = g.seq[gs][par][ph];
+      for (int ph = 0; ph < 2; ++ph)
+        for (BSeq& s : g.seqs[gs][ph]) {
           for (auto* p : s.dargs)
             if (p) (void)hipFree(p);
-          for (auto* p : s.dfin)
-            if (p) (void)hipFree(p);
+          if (s.dfin) (void)hipFree(s.dfin);
         }
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -223,29 +235,44 @@ gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
     }
   }
   for (Group& g : b->groups)
-    for (int gs = 0; gs < 2; ++gs)
-      for (int par = 0; par < 2; ++par)
-        for (int ph = 0; ph < 2; ++ph) {
-          BSeq& s = g.seq[gs][par][ph];
-          // imaginary time: the norm slot holds 1.0 whenever nothing is pending, so
-          // the renormalising first sweep is always safe
-          schedule(g.proto, gs, par, gs == 0, s.sweeps);
-          const size_t M = g.members.size();
-          s.hargs.assign(s.sweeps.size(), std::vector<SweepArgs>(M));
-          s.hfin.assign(s.sweeps.size(), std::vector<FinArgs>(M));
-          s.dargs.assign(s.sweeps.size(), nullptr);
-          s.dfin.assign(s.sweeps.size(), nullptr);
-          for (size_t k = 0; k < s.sweeps.size(); ++k) {
+    for (int gs = 0; gs < 2; ++gs) {
+      // imaginary time: the norm slot holds 1.0 whenever nothing is pending, so
+      // the renormalising first sweep is always safe
+      for (int par = 0; par < 2; ++par) schedule(g.proto, gs, par, gs == 0, g.sched[gs][par]);
+      const std::vector<Sweep>& s0 = g.sched[gs][0];
+      const std::vector<Sweep>& s1 = g.sched[gs][1];
+      for (int ph = 0; ph < 2; ++ph) {
+        std::vector<BSeq>& seqs = g.seqs[gs][ph];
+        if (mergeable(s0, s1)) {
+          seqs.resize(1);
+          seqs[0].keys = s0.size() >= s1.size() ? s0 : s1;
+          seqs[0].mask = 3;
+        } else {
+          seqs.resize(2);
+          for (int par = 0; par < 2; ++par) {
+            seqs[par].keys = g.sched[gs][par];
+            seqs[par].mask = 1 << par;
+          }
+        }
+        const size_t M = g.members.size();
+        for (BSeq& s : seqs) {
+          s.hargs.assign(s.keys.size(), std::vector<SweepArgs>(M));
+          s.dargs.assign(s.keys.size(), nullptr);
+          for (size_t k = 0; k < s.keys.size(); ++k)
             if (hipMalloc((void**)&s.dargs[k], M * sizeof(SweepArgs)) != hipSuccess) {
               gpue_batch_destroy(b);
               return GPUE_ENOMEM;
             }
-            if ((s.sweeps[k].flags & F_NORM) && hipMalloc((void**)&s.dfin[k], M * sizeof(FinArgs)) != hipSuccess) {
+          if (gs == 0) {
+            s.hfin.assign(M, FinArgs{});
+            if (hipMalloc((void**)&s.dfin, M * sizeof(FinArgs)) != hipSuccess) {
               gpue_batch_destroy(b);
               return GPUE_ENOMEM;
             }
           }
         }
+      }
+    }
   fill_args(b);
   if (upload_args(b) != GPUE_OK) {
     gpue_batch_destroy(b);

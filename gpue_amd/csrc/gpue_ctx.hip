@@ -946,6 +946,89 @@ gpue_status gpue_timing_get(gpue_ctx* c, int kind, const char** name, long* laun
   return GPUE_OK;
 }
 
+// Per-kernel cost: every distinct sweep of the gstate schedules (both parities)
+// is captured `reps` times back to back in a graph and timed with HIP events on
+// the context stream over three replays.  The launches run in place on a copy
+// of psi in the scratch buffer (the same footprint as the step, the state is
+// untouched); the norm partials are not written.
+gpue_status gpue_sweep_profile(gpue_ctx* c, int gstate, int reps, int max_kinds, int* n_kinds, const char** names,
+                               double* us_per_launch, double* launches_per_step) {
+  if (!c || (gstate != 0 && gstate != 1) || reps <= 0 || max_kinds < 0 || !n_kinds) return GPUE_EINVAL;
+  gpue_status st = apply_pending(c);
+  if (st != GPUE_OK) return st;
+  std::vector<Sweep> sched, kinds;
+  std::vector<double> count;
+  c->prof_names.clear();
+  for (int par = 0; par < 2; ++par) {
+    schedule(c, gstate, par, gstate == 0, sched);
+    for (const Sweep& sw : sched) {
+      const std::string nm = sweep_name(sw);
+      size_t k = 0;
+      while (k < c->prof_names.size() && c->prof_names[k] != nm) ++k;
+      if (k == c->prof_names.size()) {
+        c->prof_names.push_back(nm);
+        kinds.push_back(sw);
+        count.push_back(0.0);
+      }
+      count[k] += 0.5;
+    }
+  }
+  const bool timing = c->timing;
+  c->timing = false;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  const Phys ph = make_phys(c, gstate, 0.0);
+  *n_kinds = (int)kinds.size();
+  for (size_t k = 0; k < kinds.size(); ++k) {
+    Sweep sw = kinds[k];
+    sw.flags &= ~F_NORM;
+    HIPCHK(c, hipMemcpyAsync(c->tmp, c->psi, c->N * c->esz, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < reps && st == GPUE_OK; ++r) st = launch(c, sw, ph, c->tmp, c->tmp);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (st != GPUE_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      c->timing = timing;
+      return st != GPUE_OK ? st : fail(c, GPUE_EHIP, "profile capture");
+    }
+    hipGraphExec_t ge = nullptr;
+    HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    HIPCHK(c, hipGraphLaunch(ge, c->stream));   // warm-up
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < 3; ++r) HIPCHK(c, hipGraphLaunch(ge, c->stream));
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    if ((int)k < max_kinds) {
+      if (names) names[k] = c->prof_names[k].c_str();
+      if (us_per_launch) us_per_launch[k] = ms * 1000.0 / (3.0 * reps);
+      if (launches_per_step) launches_per_step[k] = count[k];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->timing = timing;
+  return GPUE_OK;
+}
+
+gpue_status gpue_math_eval(int device, int fn, const double* x, double* y, long n) {
+  if (!x || !y || n < 0 || fn < 0 || fn > 2) return GPUE_EINVAL;
+  if (n == 0) return GPUE_OK;
+  if (hipSetDevice(device) != hipSuccess) return GPUE_EHIP;
+  double* d = nullptr;
+  if (hipMalloc((void**)&d, 2 * n * sizeof(double)) != hipSuccess) return GPUE_ENOMEM;
+  hipError_t e = hipMemcpy(d, x, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_math_eval(fn, d, d + n, n, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(y, d + n, n * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? GPUE_OK : GPUE_EHIP;
+}
+
 gpue_status gpue_step_bytes(gpue_ctx* c, int gstate, double* bytes, double* sweeps) {
   if (!c) return GPUE_EINVAL;
   std::vector<Sweep> a, b;

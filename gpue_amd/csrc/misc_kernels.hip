@@ -231,6 +231,25 @@ hipError_t launch_set_double(double* p, double v, hipStream_t s) {
   return hipGetLastError();
 }
 
+// the FP64 transcendentals of the fused operators, pointwise (verification)
+__global__ void k_math_eval(int fn, const double* __restrict__ x, double* __restrict__ y, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double s, c;
+    if (fn == 0) {
+      y[i] = WMath<double>::ex(x[i]);
+    } else {
+      WMath<double>::sc(x[i], &s, &c);
+      y[i] = fn == 1 ? s : c;
+    }
+  }
+}
+
+hipError_t launch_math_eval(int fn, const double* x, double* y, long n, hipStream_t s) {
+  const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_math_eval, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, fn, x, y, n);
+  return hipGetLastError();
+}
+
 __global__ void k_fill_int(int* p, long n, int v) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }

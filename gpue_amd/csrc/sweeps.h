@@ -112,14 +112,80 @@ __device__ __forceinline__ long gid_of(const Phys& p, int xi, int yi, int zi) {
   return (long)xi + (long)p.xDim * ((long)yi + (long)p.yDim * (long)zi);
 }
 
+// FP64 exp and sincos for the fused operators.  The library versions carry
+// general special-case handling that dominates the cost of the V/2 sweeps; here
+// the arguments are finite and moderate, so a short form suffices:
+// Cody-Waite argument reduction (W. J. Cody and W. Waite, "Software Manual for
+// the Elementary Functions", 1980) followed by the Taylor polynomial of the
+// reduced argument in Horner form.  The split constants are ln 2 and pi/2
+// rounded to 42 and 3x33 significant bits, so k*C_hi is exact for every k that
+// occurs.  exp: |r| <= ln2/2, degree 13 (truncation < 5e-18 relative).
+// sin/cos: |r| <= pi/4, degrees 17 and 18 (truncation < 1e-19).  Arguments
+// outside |x| < 2^20 (sincos) go to the library; NaN propagates.
+namespace fm {
+constexpr double LN2_HI = 0.6931471805598903;
+constexpr double LN2_LO = 5.497923018708371e-14;
+constexpr double INV_LN2 = 1.4426950408889634;
+constexpr double PIO2_1 = 1.5707963267341256;
+constexpr double PIO2_2 = 6.077100506303966e-11;
+constexpr double PIO2_3 = 2.0222662487959506e-21;
+constexpr double TWO_OVER_PI = 0.6366197723675814;
+// 1/k!
+constexpr double F2 = 0.5, F3 = 0.16666666666666666, F4 = 0.041666666666666664, F5 = 0.008333333333333333,
+                 F6 = 0.001388888888888889, F7 = 0.0001984126984126984, F8 = 2.48015873015873e-05,
+                 F9 = 2.7557319223985893e-06, F10 = 2.755731922398589e-07, F11 = 2.505210838544172e-08,
+                 F12 = 2.08767569878681e-09, F13 = 1.6059043836821613e-10, F14 = 1.1470745597729725e-11,
+                 F15 = 7.647163731819816e-13, F16 = 4.779477332387385e-14, F17 = 2.8114572543455206e-15,
+                 F18 = 1.5619206968586225e-16;
+}  // namespace fm
+
+__device__ __forceinline__ double exp_cw(double x) {
+  using namespace fm;
+  x = x < -750.0 ? -750.0 : x;   // below -745.14 the result rounds to 0; comparisons keep NaN
+  x = x > 710.0 ? 710.0 : x;
+  const double k = rint(x * INV_LN2);
+  double r = fma(-k, LN2_HI, x);
+  r = fma(-k, LN2_LO, r);
+  double p = F13;
+  p = fma(p, r, F12); p = fma(p, r, F11); p = fma(p, r, F10); p = fma(p, r, F9); p = fma(p, r, F8);
+  p = fma(p, r, F7); p = fma(p, r, F6); p = fma(p, r, F5); p = fma(p, r, F4); p = fma(p, r, F3);
+  p = fma(p, r, F2); p = fma(p, r, 1.0); p = fma(p, r, 1.0);
+  return ldexp(p, (int)k);
+}
+
+__device__ __forceinline__ void sincos_cw(double x, double* s, double* c) {
+  using namespace fm;
+  if (!(fabs(x) < 1048576.0)) {   // large, infinite or NaN: library
+    sincos(x, s, c);
+    return;
+  }
+  const double q = rint(x * TWO_OVER_PI);
+  double r = fma(-q, PIO2_1, x);
+  r = fma(-q, PIO2_2, r);
+  r = fma(-q, PIO2_3, r);
+  const double z = r * r;
+  double ps = -F17;
+  ps = fma(ps, z, F15); ps = fma(ps, z, -F13); ps = fma(ps, z, F11); ps = fma(ps, z, -F9);
+  ps = fma(ps, z, F7); ps = fma(ps, z, -F5); ps = fma(ps, z, F3);
+  const double sn = fma(-r * z, ps, r);   // r - r^3/3! + ...
+  double pc = F18;
+  pc = fma(pc, z, -F16); pc = fma(pc, z, F14); pc = fma(pc, z, -F12); pc = fma(pc, z, F10);
+  pc = fma(pc, z, -F8); pc = fma(pc, z, F6); pc = fma(pc, z, -F4); pc = fma(pc, z, F2);
+  const double cs = fma(-z, pc, 1.0);     // 1 - r^2/2! + ...
+  const int n = ((int)q) & 3;
+  const double s0 = (n & 1) ? cs : sn, c0 = (n & 1) ? sn : cs;
+  *s = (n & 2) ? -s0 : s0;
+  *c = ((n + 1) & 2) ? -c0 : c0;
+}
+
 // Transcendentals in the working precision: FP32 sweeps use the FP32 library
-// functions, FP64 sweeps the FP64 ones.
+// functions, FP64 sweeps the reduced forms above.
 template <typename T>
 struct WMath;
 template <>
 struct WMath<double> {
-  static __device__ __forceinline__ double ex(double a) { return exp(a); }
-  static __device__ __forceinline__ void sc(double a, double* s, double* c) { sincos(a, s, c); }
+  static __device__ __forceinline__ double ex(double a) { return exp_cw(a); }
+  static __device__ __forceinline__ void sc(double a, double* s, double* c) { sincos_cw(a, s, c); }
 };
 template <>
 struct WMath<float> {

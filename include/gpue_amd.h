@@ -142,6 +142,18 @@ gpue_status gpue_set_timing(gpue_ctx *c, int on);
 gpue_status gpue_timing_get(gpue_ctx *c, int kind, const char **name, long *launches, double *ms);
 /* Algorithmic HBM bytes of one step in the current configuration (psi sweeps). */
 gpue_status gpue_step_bytes(gpue_ctx *c, int gstate, double *bytes_per_step, double *sweeps_per_step);
+/* Per-kernel cost of the step (no reference counterpart): each distinct sweep
+ * of the gstate schedule is launched `reps` times back to back from a graph, in
+ * place on a copy of psi in the scratch buffer, and timed with HIP events on the
+ * context stream.  Fills up to max_kinds entries: kernel name (owned by the
+ * ctx, valid until the next call), microseconds per launch, launches per step
+ * (averaged over the two gauge orders); *n_kinds = number of distinct sweeps. */
+gpue_status gpue_sweep_profile(gpue_ctx *c, int gstate, int reps, int max_kinds, int *n_kinds,
+                               const char **names, double *us_per_launch, double *launches_per_step);
+/* The FP64 exp (fn 0), sin (1) and cos (2) the fused operators evaluate on the
+ * device, applied to n host values (verification of the device math; no
+ * reference counterpart). */
+gpue_status gpue_math_eval(int device, int fn, const double *x, double *y, long n);
 
 /* --- equation strings (host-side, src/dynamic.cu:187-522) --- */
 typedef struct {          /* EqnNode_gpu (include/ds.h:71-80) */
