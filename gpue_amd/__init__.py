@@ -163,10 +163,12 @@ def make_params(**kw):
 
 
 def math_eval(fn, x, device=0):
-    """Device FP64 exp / sin / cos of the fused operators (fn "exp", "sin", "cos")."""
+    """Device exp / sin / cos of the fused operators: "exp", "sin", "cos" (FP64)
+    and "sin32", "cos32" (FP32, input rounded to float)."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.empty_like(x)
-    st = lib().gpue_math_eval(device, {"exp": 0, "sin": 1, "cos": 2}[fn], _dp(x), _dp(y), x.size)
+    st = lib().gpue_math_eval(device, {"exp": 0, "sin": 1, "cos": 2, "sin32": 3, "cos32": 4}[fn], _dp(x), _dp(y),
+                              x.size)
     if st != GPUE_OK:
         raise GpueError(st, "gpue_math_eval failed")
     return y

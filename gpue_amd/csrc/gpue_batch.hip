@@ -121,8 +121,8 @@ static void fill_args(gpue_batch* b) {
               FinArgs& f = s.hfin[m];
               std::memset(&f, 0, sizeof(f));
               f.active = in;
-              f.nb = last.row ? sweep_blocks(true, n, var, c->dims[1] * c->dims[2], 1)
-                              : sweep_blocks(false, n, var, c->dims[0], last.axis == 1 ? c->dims[2] : c->dims[1]);
+              f.nb = last.row ? sweep_blocks(c->prec, true, n, var, c->dims[1] * c->dims[2], 1)
+                              : sweep_blocks(c->prec, false, n, var, c->dims[0], last.axis == 1 ? c->dims[2] : c->dims[1]);
               f.na = 1;
               f.mode = 1;
               f.scale = c->d[0] * c->d[1] * c->d[2];

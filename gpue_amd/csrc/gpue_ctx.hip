@@ -198,8 +198,8 @@ gpue_status gpue::launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void
   // second stage of the fused reductions
   const bool ksum = s.mode == (s.row ? (int)RM_KSUM : (int)CM_KSUM);
   if (ksum || (s.flags & F_NORM)) {
-    const int nb = s.row ? sweep_blocks(true, n, var, c->dims[1] * c->dims[2], 1)
-                         : sweep_blocks(false, n, var, c->dims[0], s.axis == 1 ? c->dims[2] : c->dims[1]);
+    const int nb = s.row ? sweep_blocks(c->prec, true, n, var, c->dims[1] * c->dims[2], 1)
+                         : sweep_blocks(c->prec, false, n, var, c->dims[0], s.axis == 1 ? c->dims[2] : c->dims[1]);
     const double dV = c->d[0] * c->d[1] * c->d[2];
     e = ksum ? launch_finalize(c->partials, nb, 1, 0, s.invN, c->results + 1, c->stream)
              : launch_finalize(c->partials, nb, 1, 1, dV, c->results + 0, c->stream);
@@ -428,9 +428,9 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
   // reduction scratch: the largest grid of any reducing kernel, times 4 accumulators
   size_t nb = REDUCE_BLOCKS;
   for (int v = 0; v < N_VARIANTS; ++v) {
-    nb = std::max<size_t>(nb, sweep_blocks(true, p.xDim, v, p.yDim * p.zDim, 1));
-    if (p.dimnum > 1) nb = std::max<size_t>(nb, sweep_blocks(false, p.yDim, v, p.xDim, p.zDim));
-    if (p.dimnum > 2) nb = std::max<size_t>(nb, sweep_blocks(false, p.zDim, v, p.xDim, p.yDim));
+    nb = std::max<size_t>(nb, sweep_blocks(c->prec, true, p.xDim, v, p.yDim * p.zDim, 1));
+    if (p.dimnum > 1) nb = std::max<size_t>(nb, sweep_blocks(c->prec, false, p.yDim, v, p.xDim, p.zDim));
+    if (p.dimnum > 2) nb = std::max<size_t>(nb, sweep_blocks(c->prec, false, p.zDim, v, p.xDim, p.yDim));
   }
   // default variant: narrow lines for small (latency-bound) grids, wide ones otherwise
   for (int a = 0; a < 3; ++a) c->var[a] = c->N <= (1L << 20) ? 1 : 0;
@@ -668,12 +668,44 @@ gpue_status gpue_energy(gpue_ctx* c, double* energy, double* mu) {
   return GPUE_OK;
 }
 
+// Time `reps` back-to-back launches of one sweep, captured in a graph and run in
+// place on a copy of psi in the scratch buffer (the state is untouched, the
+// footprint is the step's); returns microseconds per launch.
+static gpue_status time_sweep(gpue_ctx* c, Sweep sw, const Phys& ph, int reps, hipEvent_t e0, hipEvent_t e1,
+                              double* us) {
+  sw.flags &= ~F_NORM;   // no reduction slots are written
+  HIPCHK(c, hipMemcpyAsync(c->tmp, c->psi, c->N * c->esz, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  gpue_status st = GPUE_OK;
+  for (int r = 0; r < reps && st == GPUE_OK; ++r) st = launch(c, sw, ph, c->tmp, c->tmp);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (st != GPUE_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return st != GPUE_OK ? st : fail(c, GPUE_EHIP, "timing capture");
+  }
+  hipGraphExec_t ge = nullptr;
+  e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipGraphLaunch(ge, c->stream);   // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+  for (int r = 0; r < 2 && e == hipSuccess; ++r) e = hipGraphLaunch(ge, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (ge) (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("timing: ") + hipGetErrorString(e));
+  *us = ms * 1000.0 / (2.0 * reps);
+  return GPUE_OK;
+}
+
 // Autotuner: for every sweep of the imaginary- and real-time schedules, time
-// each kernel variant (out of place, psi -> tmp, so the state is untouched) and
-// keep, per axis, the variant with the smallest summed time.
+// each kernel variant (graph replays of back-to-back launches, so launch
+// overhead does not decide) and keep, per axis, the variant with the smallest
+// summed time.
 static gpue_status autotune(gpue_ctx* c, int reps) {
-  if (reps <= 0) reps = 5;
-  // the trial launches overwrite the reduction slots: settle a pending norm first
+  if (reps <= 0) reps = 10;
   gpue_status st0 = apply_pending(c);
   if (st0 != GPUE_OK) return st0;
   hipEvent_t e0, e1;
@@ -683,35 +715,25 @@ static gpue_status autotune(gpue_ctx* c, int reps) {
   std::vector<Sweep> sched;
   const bool timing = c->timing;
   c->timing = false;
-  for (int gstate = 0; gstate < 2; ++gstate) {
-    for (int parity = 0; parity < 2; ++parity) {
+  gpue_status st = GPUE_OK;
+  for (int gstate = 0; gstate < 2 && st == GPUE_OK; ++gstate) {
+    for (int parity = 0; parity < 2 && st == GPUE_OK; ++parity) {
       schedule(c, gstate, parity, true, sched);
       Phys ph = make_phys(c, gstate, 0.0);
       for (Sweep s : sched) {
-        for (int v = 0; v < N_VARIANTS; ++v) {
+        for (int v = 0; v < N_VARIANTS && st == GPUE_OK; ++v) {
           s.var = v;
-          gpue_status st = launch(c, s, ph, c->psi, c->tmp);   // warm-up
-          if (st != GPUE_OK) return st;
-          HIPCHK(c, hipEventRecord(e0, c->stream));
-          for (int r = 0; r < reps; ++r) {
-            st = launch(c, s, ph, c->psi, c->tmp);
-            if (st != GPUE_OK) return st;
-          }
-          HIPCHK(c, hipEventRecord(e1, c->stream));
-          HIPCHK(c, hipEventSynchronize(e1));
-          float ms = 0.f;
-          HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-          cost[s.axis][v] += ms;
+          double us = 0.0;
+          st = time_sweep(c, s, ph, reps, e0, e1, &us);
+          cost[s.axis][v] += us;
         }
       }
     }
   }
   c->timing = timing;
-  // the trial launches wrote their norms into results[0]: restore "nothing pending"
-  {
-    hipError_t e = launch_set_double(c->results, 1.0, c->stream);
-    if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
-  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != GPUE_OK) return st;
   for (int a = 0; a < 3; ++a) {
     int best = c->var[a];
     for (int v = 0; v < N_VARIANTS; ++v)
@@ -719,8 +741,6 @@ static gpue_status autotune(gpue_ctx* c, int reps) {
     c->var[a] = best;
   }
   drop_graphs(c);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   c->tuned = true;
   return GPUE_OK;
 }
@@ -980,44 +1000,23 @@ gpue_status gpue_sweep_profile(gpue_ctx* c, int gstate, int reps, int max_kinds,
   HIPCHK(c, hipEventCreate(&e1));
   const Phys ph = make_phys(c, gstate, 0.0);
   *n_kinds = (int)kinds.size();
-  for (size_t k = 0; k < kinds.size(); ++k) {
-    Sweep sw = kinds[k];
-    sw.flags &= ~F_NORM;
-    HIPCHK(c, hipMemcpyAsync(c->tmp, c->psi, c->N * c->esz, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int r = 0; r < reps && st == GPUE_OK; ++r) st = launch(c, sw, ph, c->tmp, c->tmp);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (st != GPUE_OK || e != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      c->timing = timing;
-      return st != GPUE_OK ? st : fail(c, GPUE_EHIP, "profile capture");
-    }
-    hipGraphExec_t ge = nullptr;
-    HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-    HIPCHK(c, hipGraphLaunch(ge, c->stream));   // warm-up
-    HIPCHK(c, hipEventRecord(e0, c->stream));
-    for (int r = 0; r < 3; ++r) HIPCHK(c, hipGraphLaunch(ge, c->stream));
-    HIPCHK(c, hipEventRecord(e1, c->stream));
-    HIPCHK(c, hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipGraphExecDestroy(ge);
-    (void)hipGraphDestroy(g);
-    if ((int)k < max_kinds) {
+  for (size_t k = 0; k < kinds.size() && st == GPUE_OK; ++k) {
+    double us = 0.0;
+    st = time_sweep(c, kinds[k], ph, reps, e0, e1, &us);
+    if (st == GPUE_OK && (int)k < max_kinds) {
       if (names) names[k] = c->prof_names[k].c_str();
-      if (us_per_launch) us_per_launch[k] = ms * 1000.0 / (3.0 * reps);
+      if (us_per_launch) us_per_launch[k] = us;
       if (launches_per_step) launches_per_step[k] = count[k];
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->timing = timing;
-  return GPUE_OK;
+  return st;
 }
 
 gpue_status gpue_math_eval(int device, int fn, const double* x, double* y, long n) {
-  if (!x || !y || n < 0 || fn < 0 || fn > 2) return GPUE_EINVAL;
+  if (!x || !y || n < 0 || fn < 0 || fn > 4) return GPUE_EINVAL;
   if (n == 0) return GPUE_OK;
   if (hipSetDevice(device) != hipSuccess) return GPUE_EHIP;
   double* d = nullptr;

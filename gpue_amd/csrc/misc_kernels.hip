@@ -83,12 +83,29 @@ __global__ void k_norm_sum(const cx<T>* psi, long N, double* partials) {
 // Second stage of every reduction: one workgroup sums nb partials of each of
 // na accumulators in a fixed order.  mode 0: out[a] = tot[a] * scale;
 // mode 1: out[0] = sqrt(tot[0] * scale)  (the parSum norm, src/kernels.cu:482).
-__global__ void __launch_bounds__(256) k_finalize(const double* partials, int nb, int na, int mode, double scale,
-                                                  double* out) {
+// Sum of nb partials in a fixed order: lane t adds elements t, t+B, t+2B, ...
+// in four interleaved accumulators (so four loads are in flight per lane),
+// then the block tree.  The order depends only on nb and the block size.
+__device__ __forceinline__ double fin_sum(const double* __restrict__ p, int nb) {
+  const int B = blockDim.x;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * B < nb; i += 4 * B) {
+    const double a = p[i], b = p[i + B], c = p[i + 2 * B], d = p[i + 3 * B];
+    s0 += a;
+    s1 += b;
+    s2 += c;
+    s3 += d;
+  }
+  for (; i < nb; i += B) s0 += p[i];
+  return (s0 + s1) + (s2 + s3);
+}
+
+__global__ void __launch_bounds__(1024) k_finalize(const double* partials, int nb, int na, int mode, double scale,
+                                                   double* out) {
   __shared__ double sh[64];
   for (int a = 0; a < na; ++a) {
-    double acc[1] = {0.0};
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc[0] += partials[(size_t)a * nb + i];
+    double acc[1] = {fin_sum(partials + (size_t)a * nb, nb)};
     block_sum<1>(acc, sh);
     if (threadIdx.x == 0) out[a] = mode == 1 ? sqrt(acc[0] * scale) : acc[0] * scale;
     __syncthreads();
@@ -235,11 +252,15 @@ hipError_t launch_set_double(double* p, double v, hipStream_t s) {
 __global__ void k_math_eval(int fn, const double* __restrict__ x, double* __restrict__ y, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     double s, c;
+    float sf, cf;
     if (fn == 0) {
       y[i] = WMath<double>::ex(x[i]);
-    } else {
+    } else if (fn <= 2) {
       WMath<double>::sc(x[i], &s, &c);
       y[i] = fn == 1 ? s : c;
+    } else {
+      WMath<float>::sc((float)x[i], &sf, &cf);
+      y[i] = fn == 3 ? sf : cf;
     }
   }
 }
@@ -290,8 +311,7 @@ __global__ void __launch_bounds__(256) k_finalize_b(const FinArgs* __restrict__ 
   if (!f.active) return;
   __shared__ double sh[64];
   for (int a = 0; a < f.na; ++a) {
-    double acc[1] = {0.0};
-    for (int i = threadIdx.x; i < f.nb; i += blockDim.x) acc[0] += f.partials[(size_t)a * f.nb + i];
+    double acc[1] = {fin_sum(f.partials + (size_t)a * f.nb, f.nb)};
     block_sum<1>(acc, sh);
     if (threadIdx.x == 0) f.out[a] = f.mode == 1 ? sqrt(acc[0] * f.scale) : acc[0] * f.scale;
     __syncthreads();
@@ -304,7 +324,8 @@ hipError_t launch_finalize_batched(const FinArgs* d, int ninst, hipStream_t s) {
 }
 
 hipError_t launch_finalize(const double* partials, int nb, int na, int mode, double scale, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nb, na, mode, scale, out);
+  const int bs = nb >= 4096 ? 1024 : 256;   // block_sum takes up to 16 waves
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(bs), 0, s, partials, nb, na, mode, scale, out);
   return hipGetLastError();
 }
 

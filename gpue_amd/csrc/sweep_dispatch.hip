@@ -55,17 +55,20 @@ hipError_t launch_sweep_batched(int prec, bool row, int N, int var, int mode, co
 #undef GPUE_CASE
 }
 
-int sweep_blocks(bool row, int N, int var, int lines_or_xdim, int others) {
+int sweep_blocks(int prec, bool row, int N, int var, int lines_or_xdim, int others) {
+#define GPUE_COLB(N_, V_) \
+  (prec == 64 ? col_blocks<double, N_, V_>(lines_or_xdim, others) : col_blocks<float, N_, V_>(lines_or_xdim, others))
 #define GPUE_CASE(N_)                                                                              \
   case N_:                                                                                         \
-    if (var == 0) return row ? row_blocks<N_, 0>(lines_or_xdim) : col_blocks<N_, 0>(lines_or_xdim, others); \
-    return row ? row_blocks<N_, 1>(lines_or_xdim) : col_blocks<N_, 1>(lines_or_xdim, others);
+    if (var == 0) return row ? row_blocks<N_, 0>(lines_or_xdim) : GPUE_COLB(N_, 0);                \
+    return row ? row_blocks<N_, 1>(lines_or_xdim) : GPUE_COLB(N_, 1);
   switch (N) {
     GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
     GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
     default: return 0;
   }
 #undef GPUE_CASE
+#undef GPUE_COLB
 }
 
 }  // namespace gpue

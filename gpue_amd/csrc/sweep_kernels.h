@@ -14,14 +14,16 @@ struct RowCfg {
   static constexpr int LPW = WG / TPL;
 };
 
-// Strided sweeps read TC adjacent x-lines per row of the tile: at least 8
-// (one 128-byte line of complex doubles) and up to 256 lanes' worth for short
-// transforms, within 1024 lanes per workgroup.  Measured at 256^3 FP64 the
-// 8-column tiles run the strided FFT at copy speed; 2-4 columns lose 40-70 %.
-template <int N, int V>
+// Strided sweeps read TC adjacent x-lines per row of the tile: at least one
+// 128-byte cache line (8 complex doubles, 16 complex floats) and up to 256
+// lanes' worth for short transforms, within 1024 lanes per workgroup.
+// Measured at 256^3: FP64 8-column tiles run the strided FFT at copy speed
+// (2-4 columns lose 40-70 %); FP32 8-column tiles fetch every line twice.
+template <typename T, int N, int V>
 struct ColCfg {
   static constexpr int TPL = Plan<N, V>::TPL;
-  static constexpr int TCa = 256 / TPL > 8 ? 256 / TPL : 8;
+  static constexpr int TCmin = 128 / (2 * (int)sizeof(T));
+  static constexpr int TCa = 256 / TPL > TCmin ? 256 / TPL : TCmin;
   static constexpr int TCb = TCa > 64 ? 64 : TCa;
   static constexpr int TC = TPL * TCb > 1024 ? 1024 / TPL : TCb;
   static constexpr int WG = TPL * TC;
@@ -62,6 +64,9 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
   constexpr int RF0 = Seq<N, V, ORD_FWD>::first(), RFL = Seq<N, V, ORD_FWD>::last();
   cx<T> v[E];
   double acc = 0.0;
+  // grid indices of register slot r in the boundary layouts of the first / last radix
+  auto pos0 = [&](int r) { return Pt{bpos<N, TPL, RF0>(tid, r), yi, zi}; };
+  auto posL = [&](int r) { return Pt{bpos<N, TPL, RFL>(tid, r), yi, zi}; };
 
   if constexpr (MODE == RM_INV) {
     if (active) {
@@ -69,14 +74,10 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(tid, r)];
     }
     fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
-    if (flags & F_HALFV_POST) {
-#pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = op_halfV(p, v[r], bpos<N, TPL, RF0>(tid, r), yi, zi);
-    }
+    if (flags & F_HALFV_POST) halfV_all<T, E>(p, v, pos0);
     if (flags & F_GAUGE_POST) {
       fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
-#pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = op_A(p, v[r], 0, bpos<N, TPL, RFL>(tid, r), yi, zi, a.invN);
+      gauge_all<T, E, 0>(p, v, posL, a.invN);
       fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
     }
   } else {
@@ -90,28 +91,19 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 #pragma unroll
         for (int r = 0; r < E; ++r) v[r] = scale_norm(v[r], inv);
       }
-      if (flags & F_HALFV_PRE) {
-#pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = op_halfV(p, v[r], bpos<N, TPL, RF0>(tid, r), yi, zi);
-      }
+      if (flags & F_HALFV_PRE) halfV_all<T, E>(p, v, pos0);
     }
     fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
     if constexpr (MODE == RM_GAUGE) {
-#pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = op_A(p, v[r], 0, bpos<N, TPL, RFL>(tid, r), yi, zi, a.invN);
+      gauge_all<T, E, 0>(p, v, posL, a.invN);
       fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
     } else if constexpr (MODE == RM_FULL) {
-#pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = op_K(p, v[r], bpos<N, TPL, RFL>(tid, r), yi, zi, a.invN);
+      kstep_all<T, E>(p, v, posL, a.invN);
       fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
-      if (flags & F_HALFV_POST) {
-#pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = op_halfV(p, v[r], bpos<N, TPL, RF0>(tid, r), yi, zi);
-      }
+      if (flags & F_HALFV_POST) halfV_all<T, E>(p, v, pos0);
       if (flags & F_GAUGE_POST) {
         fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
-#pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = op_A(p, v[r], 0, bpos<N, TPL, RFL>(tid, r), yi, zi, a.invN);
+        gauge_all<T, E, 0>(p, v, posL, a.invN);
         fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
       }
     } else if constexpr (MODE == RM_KSUM) {
@@ -153,7 +145,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 template <typename T, int N, int V, int MODE>
 __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) {
   using P = Plan<N, V>;
-  constexpr int TC = ColCfg<N, V>::TC, E = P::E, TPL = P::TPL;
+  constexpr int TC = ColCfg<T, N, V>::TC, E = P::E, TPL = P::TPL;
   __shared__ cx<T> lds[P::NP > 1 ? TC * N : 1];
   __shared__ double sh[64];
   const Phys& p = a.ph;
@@ -209,12 +201,17 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
         }
       }
     } else {
-#pragma unroll
-      for (int r = 0; r < E; ++r) {
-        int yi, zi;
-        yz(bpos<N, TPL, RFL>(tid, r), yi, zi);
-        if constexpr (MODE == CM_K) v[r] = op_K(p, v[r], xi, yi, zi, a.invN);
-        else v[r] = op_A(p, v[r], a.axis, xi, yi, zi, a.invN);
+      auto posL = [&](int r) {
+        const int e = bpos<N, TPL, RFL>(tid, r);
+        return a.axis == 1 ? Pt{xi, e, o} : Pt{xi, o, e};
+      };
+      if constexpr (MODE == CM_K) {
+        auto epos = [&](int r) { return bpos<N, TPL, RFL>(tid, r); };
+        if (a.axis == 1) kstep_col<T, E, 1>(p, v, xi, o, epos, a.invN);
+        else kstep_col<T, E, 2>(p, v, xi, o, epos, a.invN);
+      } else {
+        if (a.axis == 1) gauge_all<T, E, 1>(p, v, posL, a.invN);
+        else gauge_all<T, E, 2>(p, v, posL, a.invN);
       }
       fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
       if (active) {
@@ -238,7 +235,7 @@ __global__ void __launch_bounds__((RowCfg<N, V>::WG)) row_sweep(SweepArgs a) {
   row_body<T, N, V, MODE>(a, blockIdx.x, gridDim.x);
 }
 template <typename T, int N, int V, int MODE>
-__global__ void __launch_bounds__((ColCfg<N, V>::WG)) col_sweep(SweepArgs a) {
+__global__ void __launch_bounds__((ColCfg<T, N, V>::WG)) col_sweep(SweepArgs a) {
   col_body<T, N, V, MODE>(a, blockIdx.x, gridDim.x);
 }
 
@@ -251,7 +248,7 @@ __global__ void __launch_bounds__((RowCfg<N, V>::WG)) row_sweep_b(const SweepArg
   row_body<T, N, V, MODE>(a, blockIdx.x % bpi, bpi);
 }
 template <typename T, int N, int V, int MODE>
-__global__ void __launch_bounds__((ColCfg<N, V>::WG)) col_sweep_b(const SweepArgs* __restrict__ args, int bpi) {
+__global__ void __launch_bounds__((ColCfg<T, N, V>::WG)) col_sweep_b(const SweepArgs* __restrict__ args, int bpi) {
   const SweepArgs& a = args[blockIdx.x / bpi];
   if (!a.active) return;
   col_body<T, N, V, MODE>(a, blockIdx.x % bpi, bpi);

@@ -6,8 +6,10 @@ namespace gpue {
 
 template <int N, int V>
 inline int row_blocks(int lines) { return (lines + RowCfg<N, V>::LPW - 1) / RowCfg<N, V>::LPW; }
-template <int N, int V>
-inline int col_blocks(int xDim, int others) { return ((xDim + ColCfg<N, V>::TC - 1) / ColCfg<N, V>::TC) * others; }
+template <typename T, int N, int V>
+inline int col_blocks(int xDim, int others) {
+  return ((xDim + ColCfg<T, N, V>::TC - 1) / ColCfg<T, N, V>::TC) * others;
+}
 
 template <typename T, int N, int V>
 hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
@@ -26,7 +28,7 @@ hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
 template <typename T, int N, int V>
 hipError_t launch_col_t(int mode, const SweepArgs& a, hipStream_t s) {
   const int others = a.axis == 1 ? a.ph.zDim : a.ph.yDim;
-  dim3 grid(col_blocks<N, V>(a.ph.xDim, others)), block(ColCfg<N, V>::WG);
+  dim3 grid(col_blocks<T, N, V>(a.ph.xDim, others)), block(ColCfg<T, N, V>::WG);
   switch (mode) {
     case CM_FWD: hipLaunchKernelGGL((col_sweep<T, N, V, CM_FWD>), grid, block, 0, s, a); break;
     case CM_INV: hipLaunchKernelGGL((col_sweep<T, N, V, CM_INV>), grid, block, 0, s, a); break;
@@ -55,8 +57,8 @@ hipError_t launch_row_b_t(int mode, const SweepArgs* d, int ninst, int lines, hi
 
 template <typename T, int N, int V>
 hipError_t launch_col_b_t(int mode, const SweepArgs* d, int ninst, int xDim, int others, hipStream_t s) {
-  const int bpi = col_blocks<N, V>(xDim, others);
-  dim3 grid(bpi * ninst), block(ColCfg<N, V>::WG);
+  const int bpi = col_blocks<T, N, V>(xDim, others);
+  dim3 grid(bpi * ninst), block(ColCfg<T, N, V>::WG);
   switch (mode) {
     case CM_FWD: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_FWD>), grid, block, 0, s, d, bpi); break;
     case CM_INV: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_INV>), grid, block, 0, s, d, bpi); break;

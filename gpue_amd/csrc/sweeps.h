@@ -153,12 +153,10 @@ __device__ __forceinline__ double exp_cw(double x) {
   return ldexp(p, (int)k);
 }
 
-__device__ __forceinline__ void sincos_cw(double x, double* s, double* c) {
+// core for |x| < 2^20 (no branch, so the polynomial chains of several points
+// can be interleaved); sincos_cw adds the library path for other arguments
+__device__ __forceinline__ void sincos_core(double x, double* s, double* c) {
   using namespace fm;
-  if (!(fabs(x) < 1048576.0)) {   // large, infinite or NaN: library
-    sincos(x, s, c);
-    return;
-  }
   const double q = rint(x * TWO_OVER_PI);
   double r = fma(-q, PIO2_1, x);
   r = fma(-q, PIO2_2, r);
@@ -178,73 +176,246 @@ __device__ __forceinline__ void sincos_cw(double x, double* s, double* c) {
   *c = ((n + 1) & 2) ? -c0 : c0;
 }
 
-// Transcendentals in the working precision: FP32 sweeps use the FP32 library
-// functions, FP64 sweeps the reduced forms above.
+__device__ __forceinline__ bool sincos_fast_ok(double x) { return fabs(x) < 1048576.0; }
+
+__device__ __forceinline__ void sincos_cw(double x, double* s, double* c) {
+  if (!sincos_fast_ok(x)) {   // large, infinite or NaN: library
+    sincos(x, s, c);
+    return;
+  }
+  sincos_core(x, s, c);
+}
+
+// FP32 sincos in the same form: pi/2 split into 12+12 significant bits and a
+// rounded tail (q*P1, q*P2 exact for |q| < 2^12), Taylor degrees 9 and 10 on
+// |r| <= pi/4 (truncation < 2e-9, below FP32 rounding); |x| >= 6000, infinities
+// and NaN go to the library.
+__device__ __forceinline__ void sincos_core(float x, float* s, float* c) {
+  const float q = rintf(x * 0.6366197466850281f);
+  float r = fmaf(-q, 1.57080078125f, x);
+  r = fmaf(-q, -4.453584551811218e-06f, r);
+  r = fmaf(-q, -8.705515752716053e-10f, r);
+  const float z = r * r;
+  float ps = 2.7557318844628753e-06f;
+  ps = fmaf(ps, z, -0.00019841270113829523f);
+  ps = fmaf(ps, z, 0.008333333767950535f);
+  ps = fmaf(ps, z, -0.1666666716337204f);
+  const float sn = fmaf(r * z, ps, r);
+  float pc = -2.755731998149713e-07f;
+  pc = fmaf(pc, z, 2.4801587642286904e-05f);
+  pc = fmaf(pc, z, -0.0013888889225199819f);
+  pc = fmaf(pc, z, 0.0416666679084301f);
+  pc = fmaf(pc, z, -0.5f);
+  const float cs = fmaf(z, pc, 1.0f);
+  const int n = ((int)q) & 3;
+  const float s0 = (n & 1) ? cs : sn, c0 = (n & 1) ? sn : cs;
+  *s = (n & 2) ? -s0 : s0;
+  *c = ((n + 1) & 2) ? -c0 : c0;
+}
+
+__device__ __forceinline__ bool sincos_fast_ok(float x) { return fabsf(x) < 6000.0f; }
+
+__device__ __forceinline__ void sincos_cw(float x, float* s, float* c) {
+  if (!sincos_fast_ok(x)) {
+    sincosf(x, s, c);
+    return;
+  }
+  sincos_core(x, s, c);
+}
+
+// Transcendentals in the working precision: FP64 sweeps use the reduced forms
+// above; FP32 sweeps the FP32 library exp and the FP32 reduced sincos.
 template <typename T>
 struct WMath;
 template <>
 struct WMath<double> {
   static __device__ __forceinline__ double ex(double a) { return exp_cw(a); }
   static __device__ __forceinline__ void sc(double a, double* s, double* c) { sincos_cw(a, s, c); }
+  static __device__ __forceinline__ void sc_core(double a, double* s, double* c) { sincos_core(a, s, c); }
+  static __device__ __forceinline__ bool sc_ok(double a) { return sincos_fast_ok(a); }
 };
 template <>
 struct WMath<float> {
   static __device__ __forceinline__ float ex(float a) { return expf(a); }
-  static __device__ __forceinline__ void sc(float a, float* s, float* c) { sincosf(a, s, c); }
+  static __device__ __forceinline__ void sc(float a, float* s, float* c) { sincos_cw(a, s, c); }
+  static __device__ __forceinline__ void sc_core(float a, float* s, float* c) { sincos_core(a, s, c); }
+  static __device__ __forceinline__ bool sc_ok(float a) { return sincos_fast_ok(a); }
+};
+
+// ---------------------------------------------------------------------------
+// Batched forms of the pointwise operators for the E points a lane holds.  All
+// table loads of the E points are issued before any arithmetic and every
+// branch on a field kind or gstate is taken once, outside the point loop:
+// per-point branches around loads serialise one memory round trip per point.
+// pos(r) gives the (x, y, z) grid indices of register slot r.
+// ---------------------------------------------------------------------------
+struct Pt {
+  int x, y, z;
 };
 
 // psi * exp(-arg) (imaginary time) or psi * exp(-i arg) (real time), times s
-template <typename T>
-__device__ __forceinline__ cx<T> apply_phase(cx<T> v, double arg, int gstate, double s) {
+template <typename T, int E>
+__device__ __forceinline__ void apply_phase_all(cx<T>* v, const double (&arg)[E], int gstate, double s) {
   if (gstate == 0) {
-    T f = WMath<T>::ex((T)-arg) * (T)s;
-    return cscale(v, f);
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const T f = WMath<T>::ex((T)-arg[r]) * (T)s;
+      v[r] = cscale(v[r], f);
+    }
+  } else {
+    // one range check for the lane's E arguments, then branch-free cores
+    bool fast = true;
+#pragma unroll
+    for (int r = 0; r < E; ++r) fast = fast && WMath<T>::sc_ok((T)-arg[r]);
+    T sn[E], cs[E];
+    if (fast) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) WMath<T>::sc_core((T)-arg[r], &sn[r], &cs[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < E; ++r) WMath<T>::sc((T)-arg[r], &sn[r], &cs[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = cmul(mk((T)(cs[r] * (T)s), (T)(sn[r] * (T)s)), v[r]);
   }
-  T sn, cs;
-  WMath<T>::sc((T)-arg, &sn, &cs);
-  cx<T> w = mk((T)(cs * (T)s), (T)(sn * (T)s));
-  return cmul(w, v);
 }
 
-// V/2 with the nonlinear density term: cMultDensity (src/kernels.cu:365-388) with
-// GV/EV (src/operators.cu:1081,1100) merged into one exponential.
-template <typename T>
-__device__ __forceinline__ cx<T> op_halfV(const Phys& p, cx<T> v, int xi, int yi, int zi) {
-  long gid = gid_of(p, xi, yi, zi);
-  double V = V_val(p, xi, yi, zi, gid);
-  double arg = V * p.cV;
+// V/2 with the nonlinear density term: cMultDensity (src/kernels.cu:365-388)
+// with GV/EV (src/operators.cu:1081,1100) merged into one exponential.
+template <typename T, int E, typename F>
+__device__ __forceinline__ void halfV_all(const Phys& p, cx<T>* v, F pos) {
+  double arg[E];
+  if (p.v_kind == V_SEP) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const Pt q = pos(r);
+      arg[r] = p.vs[0][q.x] + p.vs[1][q.y] + p.vs[2][q.z];
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const Pt q = pos(r);
+      arg[r] = p.Vt[gid_of(p, q.x, q.y, q.z)];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < E; ++r) arg[r] = arg[r] * p.cV;
   if (p.gpe) {
-    double d = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-    arg += p.g * d * p.cH;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const double d = (double)v[r].x * (double)v[r].x + (double)v[r].y * (double)v[r].y;
+      arg[r] += p.g * d * p.cH;
+    }
   }
-  return apply_phase(v, arg, p.gstate, 1.0);
+  apply_phase_all<T, E>(v, arg, p.gstate, 1.0);
 }
 
-// K step: GK/EK (src/operators.cu:1082,1102) with both 1/sqrt(N) of src/evolution.cu:761,777.
-// For the separable K the operator is the product of per-axis factors (no
-// transcendental per point).
-template <typename T>
-__device__ __forceinline__ cx<T> op_K(const Phys& p, cx<T> v, int xi, int yi, int zi, double invN) {
+// gauge operator along AXIS: GpA*/EpA* (src/operators.cu:1089-1110) with both
+// 1/sqrt(n) of the axis transforms folded into invn.
+template <typename T, int E, int AXIS, typename F>
+__device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double invn) {
+  double arg[E];
+  if (p.a_kind == A_TABLE) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const Pt q = pos(r);
+      arg[r] = p.At[AXIS][gid_of(p, q.x, q.y, q.z)];
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const Pt q = pos(r);
+      double A = 0.0;   // A_ZERO, and A_z of the rotation
+      if (p.a_kind == A_ROTATION) {
+        if constexpr (AXIS == 0) A = -p.yg[q.y] * p.omega * p.omegaX;
+        if constexpr (AXIS == 1) A = p.xg[q.x] * p.omega * p.omegaY;
+      }
+      arg[r] = A;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const Pt q = pos(r);
+    const double k = AXIS == 0 ? p.pxg[q.x] : AXIS == 1 ? p.pyg[q.y] : p.pzg[q.z];
+    arg[r] = arg[r] * k * p.Dt;
+  }
+  apply_phase_all<T, E>(v, arg, p.gstate, invn);
+}
+
+// K step: GK/EK (src/operators.cu:1082,1102) with both 1/sqrt(N) of
+// src/evolution.cu:761,777 folded into invN.  For the separable K the operator
+// is the product of per-axis factors (no transcendental per point).
+template <typename T, int E, typename F>
+__device__ __forceinline__ void kstep_all(const Phys& p, cx<T>* v, F pos, double invN) {
   if (p.k_kind == K_SEP) {
-    const double* fx = p.kf[0] + 2 * xi;
-    const double* fy = p.kf[1] + 2 * yi;
-    const double* fz = p.kf[2] + 2 * zi;
-    if (p.gstate == 0) return cscale(v, (T)(fx[0] * fy[0] * fz[0] * invN));
-    const double ar = fx[0] * fy[0] - fx[1] * fy[1], ai = fx[0] * fy[1] + fx[1] * fy[0];
-    const double br = (ar * fz[0] - ai * fz[1]) * invN, bi = (ar * fz[1] + ai * fz[0]) * invN;
-    return cmul(mk((T)br, (T)bi), v);
+    double f[E][6];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const Pt q = pos(r);
+      const double* fx = p.kf[0] + 2 * q.x;
+      const double* fy = p.kf[1] + 2 * q.y;
+      const double* fz = p.kf[2] + 2 * q.z;
+      f[r][0] = fx[0]; f[r][1] = fx[1]; f[r][2] = fy[0]; f[r][3] = fy[1]; f[r][4] = fz[0]; f[r][5] = fz[1];
+    }
+    if (p.gstate == 0) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) v[r] = cscale(v[r], (T)(f[r][0] * f[r][2] * f[r][4] * invN));
+    } else {
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        const double ar = f[r][0] * f[r][2] - f[r][1] * f[r][3], ai = f[r][0] * f[r][3] + f[r][1] * f[r][2];
+        const double br = (ar * f[r][4] - ai * f[r][5]) * invN, bi = (ar * f[r][5] + ai * f[r][4]) * invN;
+        v[r] = cmul(mk((T)br, (T)bi), v[r]);
+      }
+    }
+    return;
   }
-  long gid = gid_of(p, xi, yi, zi);
-  return apply_phase(v, K_val(p, xi, yi, zi, gid) * p.cK, p.gstate, invN);
+  double arg[E];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const Pt q = pos(r);
+    arg[r] = p.Kt[gid_of(p, q.x, q.y, q.z)] * p.cK;
+  }
+  apply_phase_all<T, E>(v, arg, p.gstate, invN);
 }
 
-// gauge operator along `axis`: GpA*/EpA* (src/operators.cu:1089-1110) with both 1/sqrt(n)
-template <typename T>
-__device__ __forceinline__ cx<T> op_A(const Phys& p, cx<T> v, int axis, int xi, int yi, int zi, double invn) {
-  long gid = gid_of(p, xi, yi, zi);
-  double k = axis == 0 ? p.pxg[xi] : axis == 1 ? p.pyg[yi] : p.pzg[zi];
-  double pA = A_val(p, axis, xi, yi, zi, gid) * k;
-  return apply_phase(v, pA * p.Dt, p.gstate, invn);
+// K step on a strided line (axis AXIS = 1 or 2 varies along the line; x = xi
+// and the other coordinate o are fixed): the fixed axes' factors are loaded
+// once, only the varying axis' factor per point.  Same products as kstep_all.
+template <typename T, int E, int AXIS, typename F>
+__device__ __forceinline__ void kstep_col(const Phys& p, cx<T>* v, int xi, int o, F epos, double invN) {
+  if (p.k_kind != K_SEP) {
+    kstep_all<T, E>(p, v, [&](int r) { const int e = epos(r); return AXIS == 1 ? Pt{xi, e, o} : Pt{xi, o, e}; },
+                    invN);
+    return;
+  }
+  const double fx0 = p.kf[0][2 * xi], fx1 = p.kf[0][2 * xi + 1];
+  const double* fop = p.kf[AXIS == 1 ? 2 : 1] + 2 * o;
+  const double fo0 = fop[0], fo1 = fop[1];
+  const double* fvp = p.kf[AXIS];
+  double fe[E][2];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int e = epos(r);
+    fe[r][0] = fvp[2 * e];
+    fe[r][1] = fvp[2 * e + 1];
+  }
+  if (p.gstate == 0) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const double fy0 = AXIS == 1 ? fe[r][0] : fo0, fz0 = AXIS == 1 ? fo0 : fe[r][0];
+      v[r] = cscale(v[r], (T)(fx0 * fy0 * fz0 * invN));
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const double fy0 = AXIS == 1 ? fe[r][0] : fo0, fy1 = AXIS == 1 ? fe[r][1] : fo1;
+      const double fz0 = AXIS == 1 ? fo0 : fe[r][0], fz1 = AXIS == 1 ? fo1 : fe[r][1];
+      const double ar = fx0 * fy0 - fx1 * fy1, ai = fx0 * fy1 + fx1 * fy0;
+      const double br = (ar * fz0 - ai * fz1) * invN, bi = (ar * fz1 + ai * fz0) * invN;
+      v[r] = cmul(mk((T)br, (T)bi), v[r]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -150,9 +150,10 @@ gpue_status gpue_step_bytes(gpue_ctx *c, int gstate, double *bytes_per_step, dou
  * (averaged over the two gauge orders); *n_kinds = number of distinct sweeps. */
 gpue_status gpue_sweep_profile(gpue_ctx *c, int gstate, int reps, int max_kinds, int *n_kinds,
                                const char **names, double *us_per_launch, double *launches_per_step);
-/* The FP64 exp (fn 0), sin (1) and cos (2) the fused operators evaluate on the
- * device, applied to n host values (verification of the device math; no
- * reference counterpart). */
+/* The exp (fn 0), sin (1) and cos (2) the FP64 fused operators evaluate on the
+ * device, and the sin (3) / cos (4) of the FP32 ones (input rounded to float),
+ * applied to n host values (verification of the device math; no reference
+ * counterpart). */
 gpue_status gpue_math_eval(int device, int fn, const double *x, double *y, long n);
 
 /* --- equation strings (host-side, src/dynamic.cu:187-522) --- */
