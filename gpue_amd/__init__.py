@@ -34,7 +34,7 @@ ABI_SYMBOLS = [
     "gpue_params_default", "gpue_create", "gpue_destroy", "gpue_last_error", "gpue_set_field",
     "gpue_set_field_eqn", "gpue_upload_wfc", "gpue_download_wfc", "gpue_evolve", "gpue_parsum",
     "gpue_energy", "gpue_vortex_marker_2d", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
-    "gpue_sync", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
+    "gpue_sync", "gpue_set_ramp", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
     "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_evolve", "gpue_batch_last_error", "gpue_batch_destroy",
     "gpue_server_create", "gpue_server_submit", "gpue_server_poll", "gpue_server_drain", "gpue_server_clock",
     "gpue_server_destroy",
@@ -103,6 +103,7 @@ def lib():
         L.gpue_get_grid.argtypes = [vp, ctypes.c_int, dp]
         L.gpue_set_g.argtypes = [vp, ctypes.c_double]
         L.gpue_sync.argtypes = [vp]
+        L.gpue_set_ramp.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.gpue_autotune.argtypes = [vp, ctypes.c_int, ip]
         L.gpue_set_variants.argtypes = [vp, ip]
         L.gpue_set_graph_steps.argtypes = [vp, ctypes.c_int]
@@ -326,6 +327,11 @@ class Context:
         self._check(lib().gpue_sweep_profile(self._h, gstate, reps, m, ctypes.byref(n), names, _dp(us), _dp(per)),
                     "sweep_profile")
         return [dict(name=names[k].decode(), us=float(us[k]), per_step=float(per[k])) for k in range(min(n.value, m))]
+
+    def set_ramp(self, ramp_type, num_steps):
+        """GPUE's -R ramp of the gauge exponent (0 off, 1 or 2) over an evolve
+        call of num_steps steps."""
+        self._check(lib().gpue_set_ramp(self._h, int(ramp_type), int(num_steps)), "set_ramp")
 
     def set_variants(self, variants):
         v = np.ascontiguousarray(variants, dtype=np.int32)

@@ -57,6 +57,8 @@ struct gpue_ctx {
   double* At[3] = {};
   bool az_zero = true;
   bool v_time = false;         // V is a time-dependent equation, refreshed each step
+  int ramp_type = 0;           // -R: gauge exponent scaled per step (0 off, 1, 2)
+  int ramp_steps = 0;          // numSteps of the ramped evolve call
   gpue::EqnProg veq;
   bool pending_norm = false;   // results[0] holds a norm not yet applied (else it holds 1.0)
   int next_step = 0;           // step index after the last evolve (batch parity)
@@ -83,6 +85,7 @@ struct gpue_ctx {
 namespace gpue {
 gpue_status fail(gpue_ctx* c, gpue_status s, const std::string& msg);
 Phys make_phys(const gpue_ctx* c, int gstate, double time);
+double ramp_scale(const gpue_ctx* c, int step);
 void schedule(const gpue_ctx* c, int gstate, int i, bool pending, std::vector<Sweep>& out);
 gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void* in, void* out);
 gpue_status apply_pending(gpue_ctx* c);

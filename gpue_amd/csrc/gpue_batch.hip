@@ -109,6 +109,7 @@ static void fill_args(gpue_batch* b) {
               a.axis = sw.axis;
               a.gauge_axis = sw.axis;
               a.flags = sw.flags;
+              a.tw_lds = (c->var[sw.axis] >> 1) & 1;
               a.invN = sw.invN;
               a.norm_in = c->results;
               a.partials = c->partials;
@@ -205,9 +206,9 @@ gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
   gpue_batch* b = new gpue_batch();
   for (int i = 0; i < n; ++i) {
     gpue_ctx* c = ctxs[i];
-    if (!c || c->v_time || c->p.device != ctxs[0]->p.device) {
+    if (!c || c->v_time || c->ramp_type || c->p.device != ctxs[0]->p.device) {
       gpue_batch_destroy(b);
-      return GPUE_EINVAL;   // time-dependent V and mixed devices are run per context
+      return GPUE_EINVAL;   // time-dependent V or ramp and mixed devices are run per context
     }
     b->ctx.push_back(c);
     b->active.push_back(1);

@@ -92,6 +92,19 @@ Phys gpue::make_phys(const gpue_ctx* c, int gstate, double time) {
   return f;
 }
 
+// -R ramp (src/evolution.cu:409-428, 811-832): GPUE raises the gauge operators
+// to omega_0 in place every step, so after step i they carry the exponent
+// times F_i = omega_0(0) * ... * omega_0(i): omega/(i+1) (type 1) or
+// omega*(i+1)/numSteps (type 2).  The same F_i scales the gauge exponent here,
+// in imaginary and real time (GPUE's element-wise pow of a complex phase is
+// not a phase; the exponent scaling is what the imaginary-time code computes).
+double gpue::ramp_scale(const gpue_ctx* c, int step) {
+  if (!c->ramp_type || !c->p.corotating) return 1.0;
+  const double w = c->p.omega;
+  if (c->ramp_type == 1) return w / (double)(step + 1);
+  return w * (double)(step + 1) / (double)c->ramp_steps;
+}
+
 // Sweeps of step i (src/evolution.cu:727-870).  NORM is set on the last sweep
 // of an imaginary-time step; SCALE_IN on the first when a norm is pending.
 void gpue::schedule(const gpue_ctx* c, int gstate, int i, bool pending, std::vector<Sweep>& out) {
@@ -809,7 +822,7 @@ static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gsta
     // falls inside it and the fields do not depend on time
     // (the imaginary-time graph always rescales on entry: results[0] holds the
     // pending norm or 1.0, and x*1.0 is exact)
-    if (G > 0 && !c->timing && !c->v_time && (i % 2) == 0 && nsteps - s >= G) {
+    if (G > 0 && !c->timing && !c->v_time && !c->ramp_type && (i % 2) == 0 && nsteps - s >= G) {
       bool cadence_inside = false;
       if (energy_every > 0) {
         const int nxt = ((i + energy_every - 1) / energy_every) * energy_every;
@@ -827,6 +840,7 @@ static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gsta
       }
     }
     Phys ph = make_phys(c, gstate, Dt * i);   // time = Dt*i (src/evolution.cu:408)
+    if (c->ramp_type) ph.Dt = Dt * ramp_scale(c, i);
     if (c->v_time) {
       gpue_status st = refresh_V(c, Dt * i);
       if (st != GPUE_OK) return st;
@@ -1058,6 +1072,14 @@ gpue_status gpue_eqn_parse(const char* expr, const char* const* names, const dou
 
 double gpue_eqn_eval(const gpue_eqn_node* nodes, int n, double x, double y, double z, double t) {
   return eqn_eval(reinterpret_cast<const EqnNodeC*>(nodes), n, x, y, z, t);
+}
+
+gpue_status gpue_set_ramp(gpue_ctx* c, int type, int num_steps) {
+  if (!c || type < 0 || type > 2 || (type == 2 && num_steps <= 0)) return GPUE_EINVAL;
+  c->ramp_type = type;
+  c->ramp_steps = num_steps;
+  drop_graphs(c);
+  return GPUE_OK;
 }
 
 gpue_status gpue_set_graph_steps(gpue_ctx* c, int steps) {

@@ -76,6 +76,7 @@ struct SweepArgs {
   int axis;           // 0 x (row), 1 y, 2 z
   int gauge_axis;     // axis of the A operator applied by F_GAUGE_POST / gauge modes
   unsigned flags;
+  int tw_lds;         // 1: copy the twiddle table to (dynamic) LDS first, read it there
   double invN;        // normalisation folded into K (1/N) or A (1/n_axis)
   const double* norm_in;   // pending renormalisation divisor (F_SCALE_IN)
   double* partials;        // per-workgroup partial sums (F_NORM, KSUM modes)

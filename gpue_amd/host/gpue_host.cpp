@@ -113,9 +113,11 @@ Grid parse_args(int argc, char** argv) {
   par.store("g_direct", std::nan(""));
   par.store("precision", 64);
   par.store("stale_handoff", true);
+  par.store("ramp", false);           // src/parser.cu:45-46
+  par.store("ramp_type", 1);
   optind = 1;
   int opt;
-  while ((opt = getopt(argc, argv, "b:d:C:x:y:w:m:G:g:e:T:t:n:p:rQ:L:E::lsi:X:Y:WU:V:z:A:v:Z:fc:I:j:B:M:N:P:o")) != -1) {
+  while ((opt = getopt(argc, argv, "b:d:C:x:y:w:m:G:g:e:T:t:n:p:rQ:L:E::lsi:X:Y:WU:V:z:A:v:Z:fc:I:j:B:M:N:P:oR:")) != -1) {
     switch (opt) {
       case 'x': par.store("xDim", std::atoi(optarg)); break;
       case 'y': par.store("yDim", std::atoi(optarg)); break;
@@ -175,6 +177,11 @@ Grid parse_args(int argc, char** argv) {
       case 'N': par.store("g_direct", std::atof(optarg)); break;
       case 'P': par.store("precision", std::atoi(optarg)); break;
       case 'o': par.store("stale_handoff", false); break;
+      case 'R':   // src/parser.cu:188-195
+        std::printf("Ramping omega with imaginary time evolution\n");
+        par.store("ramp", true);
+        par.store("ramp_type", std::atoi(optarg));
+        break;
       default:
         throw std::runtime_error(std::string("unsupported option -") + (char)optopt +
                                  " (analysis/experiment flags of GPUE are outside the step)");
@@ -516,8 +523,18 @@ void evolve(Grid& par, gpue_ctx* ctx, int numSteps, unsigned gstate, std::string
   const bool write_it = par.bval("write_it");
   const bool stale = par.bval("stale_handoff");
   const double mask_2d = par.dval("mask_2d");
-  const double omega_0 = par.dval("omega") * par.dval("omegaX");
-  const std::string fileName = gstate == 0 ? "wfc_0_const" : "wfc_ev";
+  const bool ramp = par.bval("ramp");
+  const int ramp_type = par.ival("ramp_type");
+  const double omega = par.dval("omega");
+  // src/evolution.cu:353,409-428: printed omega_0, the ramp's per-step factor
+  auto omega_0 = [&](int i) {
+    if (!ramp) return omega * par.dval("omegaX");
+    if (ramp_type == 1) return i == 0 ? omega : (double)i / (double)(i + 1);
+    return i == 0 ? omega / (double)numSteps : (double)(i + 1) / (double)i;
+  };
+  check(ctx, gpue_set_ramp(ctx, ramp ? ramp_type : 0, numSteps), "ramp");
+  // src/evolution.cu:445-460,705-716
+  const std::string fileName = gstate == 0 ? (ramp ? "wfc_0_ramp" : "wfc_0_const") : (ramp ? "wfc_ev_ramp" : "wfc_ev");
   const std::string efile = data_dir + (gstate == 0 ? "energyi.dat" : "energy.dat");
   std::printf("Timestep for %s set as: %E\n", gstate == 0 ? "groundstate solver" : "evolution",
               gstate == 0 ? par.dval("gdt") : par.dval("dt"));
@@ -531,7 +548,7 @@ void evolve(Grid& par, gpue_ctx* ctx, int numSteps, unsigned gstate, std::string
   while (i < numSteps && !stopped) {
     if (i % printSteps == 0) {
       check(ctx, gpue_download_wfc(ctx, host.data()), "download");
-      std::printf("Step: %d    Omega: %lf\n", i, omega_0);
+      std::printf("Step: %d    Omega: %lf\n", i, omega_0(i));
       double spent = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       std::printf("Time spent: %lf\n", spent);
       if (gstate == 1 && dimnum == 2) {   // 2D vortex tracking (src/evolution.cu:483-700)

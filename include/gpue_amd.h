@@ -119,14 +119,21 @@ gpue_status gpue_get_grid(gpue_ctx *c, int axis, double *out);
  * (src/evolution.cu:739); this is the ABI extension the quench workload needs. */
 gpue_status gpue_set_g(gpue_ctx *c, double g);
 
+/* GPUE's -R ramp (src/evolution.cu:409-428, 811-832): 0 off; 1 or 2 scale the
+ * gauge exponent of step i (counted from 0 within the ramped evolve call of
+ * num_steps steps) by omega/(i+1) or omega*(i+1)/num_steps, GPUE's cumulative
+ * omega_0.  Steps of a ramped context are launched one by one (no graphs). */
+gpue_status gpue_set_ramp(gpue_ctx *c, int type, int num_steps);
+
 /* Wait for all work on the ctx stream. */
 gpue_status gpue_sync(gpue_ctx *c);
 
 /* FFT kernel shape per axis (no reference counterpart: cuFFT plans choose their
  * own).  gpue_autotune times every variant of every sweep of the current
- * configuration out of place and keeps the fastest per axis; variants[3]
- * receives the choice (0 = wide lines, 1 = narrow lines).  gpue_set_variants
- * forces a choice. */
+ * configuration (graph replays of back-to-back launches on a scratch copy of
+ * psi) and keeps the fastest per axis; variants[3] receives the choice, a
+ * value 0..3: bit 0 selects narrow lines (fewer points per lane), bit 1 reads
+ * the twiddle table from a copy in LDS.  gpue_set_variants forces a choice. */
 gpue_status gpue_autotune(gpue_ctx *c, int reps, int *variants);
 gpue_status gpue_set_variants(gpue_ctx *c, const int *variants);
 /* Steady-state steps are captured as a HIP graph of `steps` consecutive steps

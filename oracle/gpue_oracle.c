@@ -192,6 +192,8 @@ struct or_ctx {
     int v_user;                    /* V table supplied by the caller */
     or_c *GV, *EV, *GK, *EK, *GpAx, *GpAy, *GpAz, *EpAx, *EpAy, *EpAz;
     or_c *wfc;
+    int ramp_type, ramp_steps;     /* -R: 0 off, 1 or 2 (src/evolution.cu:409-428) */
+    or_c *RpA[3];                  /* ramped gauge operators of the current step */
 };
 
 double or_dV(const or_ctx *c) { return c->dx * c->dy * c->dz; }
@@ -319,6 +321,7 @@ or_ctx *or_create(const or_params *pin) {
 
 void or_destroy(or_ctx *c) {
     if (!c) return;
+    for (int a = 0; a < 3; ++a) free(c->RpA[a]);
     free(c->x); free(c->y); free(c->z); free(c->px); free(c->py); free(c->pz);
     free(c->V); free(c->K); free(c->Ax); free(c->Ay); free(c->Az);
     free(c->GV); free(c->EV); free(c->GK); free(c->EK); free(c->GpAx); free(c->GpAy);
@@ -457,6 +460,58 @@ void or_parsum(or_ctx *c) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* -R ramp: src/evolution.cu:409-428 sets omega_0 per step, :811-832 raises  */
+/* the gauge operators to it in place, cumulatively (src/kernels.cu:491-497  */
+/* scalarPow acts on re and im separately).  Imaginary time: GpA is real and */
+/* positive, so after step i the operator is GpA^(f_0 f_1 ... f_i), kept    */
+/* here literally by repeated pow.  Real time: the element-wise pow of a    */
+/* complex phase is not a phase; the defined semantics is the same exponent */
+/* scaling, exp(-i pA dt F_i) with F_i = f_0 ... f_i.                       */
+/* ------------------------------------------------------------------------ */
+
+void or_set_ramp(or_ctx *c, int type, int num_steps) {
+    c->ramp_type = type;
+    c->ramp_steps = num_steps;
+    for (int a = 0; a < 3; ++a) {
+        if (!c->RpA[a]) c->RpA[a] = (or_c *)calloc((size_t)c->N, sizeof(or_c));
+    }
+}
+
+/* omega_0 of step i (src/evolution.cu:412-427) */
+double or_ramp_factor(int type, double omega, int i, int num_steps) {
+    if (type == 1) return i == 0 ? omega : (double)i / (double)(i + 1);
+    return i == 0 ? omega / (double)num_steps : (double)(i + 1) / (double)i;
+}
+
+static void ramp_ops(or_ctx *c, int i, int gstate) {
+    const or_params *p = &c->p;
+    const double w0 = or_ramp_factor(c->ramp_type, p->omega, i, c->ramp_steps);
+    const or_c *G[3] = {c->GpAx, c->GpAy, c->GpAz};
+    if (gstate == 0) {
+        for (int a = 0; a < 3; ++a) {
+            or_c *R = c->RpA[a];
+            if (i == 0) memcpy(R, G[a], sizeof(or_c) * (size_t)c->N);
+            #pragma omp parallel for schedule(static)
+            for (long gid = 0; gid < c->N; ++gid) { R[gid].x = pow(R[gid].x, w0); R[gid].y = pow(R[gid].y, w0); }
+        }
+        return;
+    }
+    double F = 1.0;
+    for (int j = 0; j <= i; ++j) F *= or_ramp_factor(c->ramp_type, p->omega, j, c->ramp_steps);
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int zi = 0; zi < p->zDim; ++zi)
+        for (int yi = 0; yi < p->yDim; ++yi)
+            for (int xi = 0; xi < p->xDim; ++xi) {
+                long gid = GID(xi, yi, zi);
+                double pA[3] = {c->Ax[gid] * c->px[xi], c->Ay[gid] * c->py[yi], c->Az[gid] * c->pz[zi]};
+                for (int a = 0; a < 3; ++a) {
+                    c->RpA[a][gid].x = cos(-pA[a] * (p->dt * F));
+                    c->RpA[a][gid].y = sin(-pA[a] * (p->dt * F));
+                }
+            }
+}
+
+/* ------------------------------------------------------------------------ */
 /* One step: src/evolution.cu:727-870                                        */
 /* ------------------------------------------------------------------------ */
 
@@ -484,6 +539,10 @@ void or_step(or_ctx *c, int i, int gstate) {
     /* gauge (:811-866); flip = i%2 (:837,842) */
     if (p->corotating) {
         int flip = i % 2;
+        if (c->ramp_type) {
+            ramp_ops(c, i, gstate);
+            for (int a = 0; a < 3; ++a) Aop[a] = c->RpA[a];
+        }
         if (p->dimnum == 1) {
             gauge_axis(c, 0, Aop[0]);
         } else if (p->dimnum == 2) {

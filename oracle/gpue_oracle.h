@@ -73,6 +73,11 @@ double  or_energy(or_ctx *c, double *mu_out);
 int     or_evolve(or_ctx *c, int nsteps, int gstate, int energy_every, double thr,
                   double *energies, int max_energies, int *n_energies);
 
+/* -R ramp of the gauge operators (0 off, 1 or 2; num_steps = steps of the
+ * evolve call, step indices count from 0 within it) */
+void    or_set_ramp(or_ctx *c, int type, int num_steps);
+double  or_ramp_factor(int type, double omega, int i, int num_steps);
+
 /* --- standalone kernels restated (for KATs) --- */
 void    or_parsum_raw(or_c *wfc, int dimnum, int xDim, int yDim, int zDim,
                       double dx, double dy, double dz);

@@ -65,6 +65,9 @@ def lib():
         L.or_dV.argtypes = [vp]
         L.or_dV.restype = ctypes.c_double
         L.or_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.or_set_ramp.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.or_ramp_factor.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.or_ramp_factor.restype = ctypes.c_double
         L.or_parsum.argtypes = [vp]
         L.or_energy.argtypes = [vp, dp]
         L.or_energy.restype = ctypes.c_double
@@ -205,6 +208,10 @@ class Oracle:
     def set_g(self, g):
         lib().or_set_g(self._h, g)
 
+    def set_ramp(self, ramp_type, num_steps):
+        """-R ramp (src/evolution.cu:409-428,811-832); steps count from 0."""
+        lib().or_set_ramp(self._h, int(ramp_type), int(num_steps))
+
     @property
     def dV(self):
         return lib().or_dV(self._h)
@@ -226,3 +233,8 @@ class Oracle:
         done = lib().or_evolve(self._h, nsteps, gstate, energy_every, thr, _dp(en), max_energies,
                                ctypes.byref(ne))
         return done, en[: min(ne.value, max_energies)]
+
+
+def ramp_factor(ramp_type, omega, i, num_steps):
+    """omega_0 of step i (src/evolution.cu:412-427)."""
+    return lib().or_ramp_factor(int(ramp_type), float(omega), int(i), int(num_steps))

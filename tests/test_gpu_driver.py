@@ -76,8 +76,26 @@ def test_driver_matches_oracle(tmp_path):
     assert params.startswith("[Params]\n") and "xMax=8" in params
 
 
+def test_driver_ramp_matches_oracle(tmp_path):
+    # -R 2: ramped imaginary time, GPUE's wfc_0_ramp file names (src/evolution.cu:451-453)
+    n, L, g, om, gsteps, ps = 64, 8.0, 60.0, 0.6, 40, 20
+    d = str(tmp_path)
+    cmd = [GPUE, "-d", d, "-x", str(n), "-y", str(n), "-b", str(L), "-B", "1", "-M", "1", "-X", "1", "-Y", "1",
+           "-N", str(g), "-s", "-l", "-w", str(om), "-g", str(gsteps), "-e", "0", "-p", str(ps), "-T", "1e-3",
+           "-W", "-R", "2", "-o"]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr + out.stdout
+    orc = O.Oracle(dimnum=2, xDim=n, yDim=n, xMax=L, yMax=L, g=g, gpe=1, corotating=1, omega=om, gdt=1e-3, dt=1e-4)
+    orc.set_ramp(2, gsteps)
+    orc.set_wfc(read_wfc(d, "WFC", 0, n))
+    for i in range(gsteps):
+        if i % ps == 0:
+            assert rel_l2(read_wfc(d, "wfc_0_ramp", i, n), orc.wfc()) < 1e-10
+        orc.step(i, 0)
+
+
 def test_driver_rejects_unsupported_flags(tmp_path):
-    out = subprocess.run([GPUE, "-d", str(tmp_path), "-R", "1"], capture_output=True, text=True, timeout=60)
+    out = subprocess.run([GPUE, "-d", str(tmp_path), "-K", "1"], capture_output=True, text=True, timeout=60)
     assert out.returncode != 0 and "unsupported option" in out.stderr
     out = subprocess.run([GPUE, "-d", str(tmp_path), "-x", "100", "-y", "100"], capture_output=True, text=True,
                          timeout=60)

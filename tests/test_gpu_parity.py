@@ -223,3 +223,55 @@ def test_graph_replay_matches_eager(gstate):
         assert a.energy() == b.energy()
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("name,dim,kw", [c for c in CASES if c[0] in ("1d-gauge", "2d-gauge", "3d-gauge", "2d-48",
+                                                                      "3d-96")],
+                         ids=["1d-gauge", "2d-gauge", "3d-gauge", "2d-48", "3d-96"])
+def test_every_kernel_variant(name, dim, kw, variant):
+    # the autotuner may pick any variant per axis (plan x twiddle source): each
+    # must meet the FP64 bar, imaginary and real time
+    for gstate in (0, 1):
+        ctx = G.Context(precision=64, gdt=1e-3, dt=5e-4, **kw)
+        ctx.set_variants([variant] * 3)
+        orc = O.Oracle(gdt=1e-3, dt=5e-4, **{k: v for k, v in kw.items() if k != "a_kind"})
+        psi = seeded_psi(ctx.shape, [ctx.grid(a) for a in range(ctx.dimnum)], seed=2, phase=True)
+        psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
+        ctx.upload(psi)
+        orc.set_wfc(psi)
+        ctx.evolve(10, gstate, first_step=1)
+        for i in range(1, 11):
+            orc.step(i, gstate)
+        assert rel_l2(ctx.download(), orc.wfc()) <= TOL64, (variant, gstate)
+        ctx.close()
+
+
+@pytest.mark.parametrize("ramp_type", [1, 2])
+@pytest.mark.parametrize("name,dim,kw", [c for c in CASES if c[0] in ("2d-gauge", "3d-gauge")],
+                         ids=["2d-gauge", "3d-gauge"])
+def test_ramp_matches_oracle(name, dim, kw, ramp_type):
+    # -R: the oracle raises GPUE's gauge operators to omega_0 step by step
+    # (imaginary time) or scales the phase (real time); the HIP path scales the
+    # exponent by the closed-form product.  FP64 bar.
+    for gstate in (0, 1):
+        n = 12
+        ctx = G.Context(precision=64, gdt=1e-3, dt=5e-4, **kw)
+        orc = O.Oracle(gdt=1e-3, dt=5e-4, **kw)
+        ctx.set_ramp(ramp_type, n)
+        orc.set_ramp(ramp_type, n)
+        psi = seeded_psi(ctx.shape, [ctx.grid(a) for a in range(ctx.dimnum)], seed=4, phase=True)
+        psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
+        ctx.upload(psi)
+        orc.set_wfc(psi)
+        ctx.evolve(n, gstate)
+        for i in range(n):
+            orc.step(i, gstate)
+        assert rel_l2(ctx.download(), orc.wfc()) <= TOL64, (ramp_type, gstate)
+        # the ramp changes the result (the test is not vacuous)
+        ref = O.Oracle(gdt=1e-3, dt=5e-4, **kw)
+        ref.set_wfc(psi)
+        for i in range(n):
+            ref.step(i, gstate)
+        assert rel_l2(ref.wfc(), orc.wfc()) > 1e-6
+        ctx.close()

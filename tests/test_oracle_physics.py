@@ -173,3 +173,46 @@ def test_ls_fit_linear_vortex_closed_form():
         ijw = np.array([[int(i0), int(j0), 1]], dtype=np.int32)
         xy = O.ls_fit(ijw, psi)
         assert abs(xy[0, 0] - i0) < 1e-12 and abs(xy[0, 1] - j0) < 1e-12
+
+
+def test_ramp_factors_and_cumulative_exponent():
+    # src/evolution.cu:409-428: omega_0 of step i; the operators are raised to
+    # it every step, so after step i the exponent carries the running product,
+    # which telescopes to omega/(i+1) (type 1) or omega*(i+1)/n (type 2)
+    om, n = 0.7, 9
+    for t in (1, 2):
+        prod = 1.0
+        for i in range(n):
+            f = O.ramp_factor(t, om, i, n)
+            expect = (om if i == 0 else i / (i + 1)) if t == 1 else (om / n if i == 0 else (i + 1) / i)
+            assert f == expect
+            prod *= f
+            closed = om / (i + 1) if t == 1 else om * (i + 1) / n
+            assert abs(prod - closed) <= 4e-15 * closed
+
+
+def test_ramp_imaginary_time_is_exponent_scaling():
+    # imaginary time: raising the real positive operator exp(-pA gdt) to the
+    # step factors one after another depends only on their product, i.e. the
+    # ramp scales the gauge exponent.  Two ramp types whose running products
+    # agree at step 1 (type 1: omega * 1/2; type 2, n = 4: omega/4 * 2) through
+    # different pow sequences must give the same step.
+    kw = dict(dimnum=1, xDim=64, xMax=8.0, g=20.0, gpe=1, corotating=1, omega=0.5, gdt=1e-3)
+    a = O.Oracle(**kw)
+    b = O.Oracle(**kw)
+    rng = np.random.default_rng(2)
+    psi = np.exp(-a.grid(0) ** 2 / 2) * np.exp(1j * rng.uniform(0, 2 * np.pi, 64))
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * a.dV)
+    a.set_ramp(1, 4)
+    b.set_ramp(2, 4)
+    # step 0 (factors omega and omega/4) advances the operators; step 1 is then
+    # taken from the same state
+    a.set_wfc(psi)
+    b.set_wfc(psi)
+    a.step(0, 0)
+    b.step(0, 0)
+    a.set_wfc(psi)
+    b.set_wfc(psi)
+    a.step(1, 0)
+    b.step(1, 0)
+    assert np.linalg.norm(a.wfc() - b.wfc()) <= 1e-13 * np.linalg.norm(b.wfc())
