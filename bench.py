@@ -64,6 +64,15 @@ def make_ctx(G, prec, cfg, seed=1, phase=False, noise=0.01):
     return ctx, psi
 
 
+def tune(ctx, args):
+    ctx.set_graph_steps(args.graph_steps)
+    if args.variants:
+        v = [int(a) for a in args.variants.split(",")]
+        ctx.set_variants(v)
+        return tuple(v)
+    return ctx.autotune()
+
+
 def run_schedule(ctx, imag, real):
     if imag:
         ctx.evolve(imag, 0)
@@ -137,11 +146,12 @@ def cpu_baseline(budget_s):
                       f"{steps} imaginary-time steps of the job, {el:.1f} s"}
 
 
-def batched_sweep(G, n_inst=64, seed=4, max_steps=3000, tol=1e-8):
+def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
     """64 independent 2D imaginary-time solves on uneven grids, FP32 bulk with
     FP64 norm/energy, advanced together through one batch (one launch per sweep
     per shape, two steps per graph replay) until each instance's energy changes
-    by less than tol (relative) over 100 steps."""
+    by less than tol (relative) over 100 steps (at most max_steps).  psi0 is
+    the noisy Gaussian exp(-r^2/2)(1 + 0.01 U[-1,1]), seeded per instance."""
     rng = np.random.default_rng(seed)
     sizes = rng.choice([128, 192, 256, 384, 512], n_inst)
     ctxs = []
@@ -152,7 +162,7 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=3000, tol=1e-8):
         cfg = dict(dimnum=2, xDim=n, yDim=n, xMax=L, yMax=L, g=float(rng.uniform(100, 2000)), gpe=1,
                    omega=float(rng.uniform(0, 0.95)), corotating=1, omegaX=1.0,
                    omegaY=float(rng.uniform(0.8, 1.2)), gdt=1e-3, dt=1e-4)
-        ctx, _ = make_ctx(G, 32, cfg, seed=100 + k, phase=True, noise=0.01)
+        ctx, _ = make_ctx(G, 32, cfg, seed=100 + k, phase=False, noise=0.01)
         if n not in tuned:
             tuned[n] = ctx.autotune()
         else:
@@ -258,6 +268,10 @@ def main():
                     help="profiling: run one workload only (copy: a plain 256 MiB device copy, the "
                          "calibration kernel for the HBM byte counters)")
     ap.add_argument("--steps3", type=int, default=200, help="timed 256^3 steps per precision")
+    ap.add_argument("--graph-steps", type=int, default=8,
+                    help="steps per captured graph (0: every sweep launched individually, e.g. under a tracer)")
+    ap.add_argument("--variants", type=str, default=None,
+                    help="kernel variants per axis, e.g. 1,1,1 (default: autotune)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; every solve runs on a single MI355X "
@@ -274,7 +288,7 @@ def main():
     # ---------------- headline: 512^2 rotating condensate, FP64, gauge ----------------
     if args.only in (None, "512"):
         ctx, psi0 = make_ctx(G, 64, CFG2, phase=True, noise=0.0)
-        variants = ctx.autotune()
+        variants = tune(ctx, args)
         run_schedule(ctx, W // 2, W - W // 2)
         ctx.upload(psi0)
         ctx.sync()
@@ -319,7 +333,7 @@ def main():
         res3 = {}
         for prec in (64, 32):
             ctx, psi0 = make_ctx(G, prec, CFG3, seed=1)
-            variants = ctx.autotune()
+            variants = tune(ctx, args)
             ctx.evolve(10, 0)
             ctx.upload(psi0)
             ctx.sync()

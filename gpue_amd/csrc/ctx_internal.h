@@ -87,6 +87,8 @@ gpue_status fail(gpue_ctx* c, gpue_status s, const std::string& msg);
 Phys make_phys(const gpue_ctx* c, int gstate, double time);
 double ramp_scale(const gpue_ctx* c, int step);
 void schedule(const gpue_ctx* c, int gstate, int i, bool pending, std::vector<Sweep>& out);
+void schedule_block(const gpue_ctx* c, int gstate, int first, int n, std::vector<Sweep>& out,
+                    std::vector<int>& step_of);
 gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void* in, void* out);
 gpue_status apply_pending(gpue_ctx* c);
 gpue_status evolve_async(gpue_ctx* c, int first_step, int nsteps, int gstate);

@@ -144,7 +144,8 @@ void gpue::schedule(const gpue_ctx* c, int gstate, int i, bool pending, std::vec
 }
 
 static std::string sweep_name(const Sweep& s) {
-  static const char* rm[] = {"row_fwd", "row_inv", "row_gauge", "row_full", "row_ksum"};
+  static const char* rm[] = {"row_fwd", "row_inv", "row_gauge", "row_full", "row_ksum", "row_inv_fwd",
+                             "row_gauge_fwd"};
   static const char* cm[] = {"fwd", "inv", "k", "gauge", "ksum"};
   std::string n = s.row ? rm[s.mode] : (std::string(s.axis == 1 ? "coly_" : "colz_") + cm[s.mode]);
   if (s.flags & F_HALFV_PRE) n += "+vpre";
@@ -425,15 +426,22 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
     std::vector<double> re, im;
     twiddles(n, re, im);
     void* t = nullptr;
-    if (hipMalloc(&t, (size_t)n * c->esz) != hipSuccess) return bail(GPUE_ENOMEM, "twiddles");
+    if (hipMalloc(&t, (size_t)n * 16) != hipSuccess) return bail(GPUE_ENOMEM, "twiddles");
     c->tw[n] = t;
     if (c->prec == 64) {
       std::vector<double> il(2 * n);
       for (int m = 0; m < n; ++m) { il[2 * m] = re[m]; il[2 * m + 1] = im[m]; }
       e = hipMemcpy(t, il.data(), il.size() * sizeof(double), hipMemcpyHostToDevice);
     } else {
-      std::vector<float> il(2 * n);
-      for (int m = 0; m < n; ++m) { il[2 * m] = (float)re[m]; il[2 * m + 1] = (float)im[m]; }
+      // split twiddles: (hi.re, hi.im, lo.re, lo.im), lo = the rounded remainder
+      std::vector<float> il(4 * n);
+      for (int m = 0; m < n; ++m) {
+        const float hr = (float)re[m], hi = (float)im[m];
+        il[4 * m] = hr;
+        il[4 * m + 1] = hi;
+        il[4 * m + 2] = (float)(re[m] - (double)hr);
+        il[4 * m + 3] = (float)(im[m] - (double)hi);
+      }
       e = hipMemcpy(t, il.data(), il.size() * sizeof(float), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) return bail(GPUE_EHIP, "twiddle upload");
@@ -768,20 +776,60 @@ static gpue_status refresh_V(gpue_ctx* c, double t) {
   return GPUE_OK;
 }
 
+}  // extern "C"
+
+// Sweeps of steps first..first+n-1 with, in real time, every row sweep that
+// ends a step fused with the RM_FWD that starts the next (no norm stands
+// between them, and the fields do not change): the pair's arithmetic is
+// unchanged, one read and one write of psi are saved.
+void gpue::schedule_block(const gpue_ctx* c, int gstate, int first, int n, std::vector<Sweep>& out,
+                          std::vector<int>& step_of) {
+  std::vector<Sweep> one;
+  out.clear();
+  step_of.clear();
+  for (int s = 0; s < n; ++s) {
+    schedule(c, gstate, first + s, gstate == 0, one);
+    for (const Sweep& sw : one) {
+      out.push_back(sw);
+      step_of.push_back(first + s);
+    }
+  }
+  if (gstate != 1 || c->v_time || c->ramp_type) return;
+  std::vector<Sweep> fused;
+  std::vector<int> fstep;
+  for (size_t k = 0; k < out.size(); ++k) {
+    const Sweep& a = out[k];
+    if (k + 1 < out.size() && step_of[k + 1] == step_of[k] + 1 && a.row && (a.mode == RM_INV || a.mode == RM_GAUGE) &&
+        out[k + 1].row && out[k + 1].mode == RM_FWD && !(out[k + 1].flags & F_SCALE_IN)) {
+      Sweep m = a;
+      m.mode = a.mode == RM_INV ? RM_INV_FWD : RM_GAUGE_FWD;
+      m.flags |= out[k + 1].flags & F_HALFV_PRE;
+      fused.push_back(m);
+      fstep.push_back(step_of[k]);
+      ++k;
+      continue;
+    }
+    fused.push_back(a);
+    fstep.push_back(step_of[k]);
+  }
+  out.swap(fused);
+  step_of.swap(fstep);
+}
+
+extern "C" {
+
 // Capture graph_steps steady-state steps (starting on an even step, norm
 // pending in imaginary time) into a graph for this gstate.
 static gpue_status build_graph(gpue_ctx* c, int gstate) {
   std::vector<Sweep> sched;
+  std::vector<int> step_of;
   const double Dt = gstate == 0 ? c->p.gdt : c->p.dt;
+  schedule_block(c, gstate, 0, c->graph_steps, sched, step_of);
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   gpue_status st = GPUE_OK;
-  for (int s = 0; s < c->graph_steps && st == GPUE_OK; ++s) {
-    Phys ph = make_phys(c, gstate, Dt * s);
-    schedule(c, gstate, s, gstate == 0, sched);
-    for (const Sweep& sw : sched) {
-      st = launch(c, sw, ph, c->psi, c->psi);
-      if (st != GPUE_OK) break;
-    }
+  for (size_t k = 0; k < sched.size() && st == GPUE_OK; ++k) {
+    const Phys ph = make_phys(c, gstate, Dt * step_of[k]);
+    st = launch(c, sched[k], ph, c->psi, c->psi);
   }
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);

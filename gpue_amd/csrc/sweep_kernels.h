@@ -35,12 +35,13 @@ struct ColCfg {
 // N*sizeof(cx<T>) of LDS per workgroup, so the autotuner decides per axis.
 // The first read of the table follows the first exchange barrier.
 template <typename T, int N, int NP>
-__device__ __forceinline__ const cx<T>* twiddle_source(const SweepArgs& a, int wg) {
-  const cx<T>* tw = (const cx<T>*)a.tw;
+__device__ __forceinline__ const typename TwE<T>::type* twiddle_source(const SweepArgs& a, int wg) {
+  using W = typename TwE<T>::type;
+  const W* tw = (const W*)a.tw;
   if constexpr (NP > 1) {
     if (a.tw_lds) {
       extern __shared__ __align__(16) unsigned char gpue_dyn_lds[];
-      cx<T>* t = reinterpret_cast<cx<T>*>(gpue_dyn_lds);
+      W* t = reinterpret_cast<W*>(gpue_dyn_lds);
       for (int m = threadIdx.x; m < N; m += wg) t[m] = tw[m];
       return t;
     }
@@ -78,7 +79,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
   const int yi = active ? line % p.yDim : 0, zi = active ? line / p.yDim : 0;
   const cx<T>* __restrict__ in = (const cx<T>*)a.in + (long)(active ? line : 0) * N;
   cx<T>* __restrict__ out = (cx<T>*)a.out + (long)(active ? line : 0) * N;
-  const cx<T>* tw = twiddle_source<T, N, P::NP>(a, RowCfg<N, V>::WG);
+  const auto* tw = twiddle_source<T, N, P::NP>(a, RowCfg<N, V>::WG);
   const LdsRow la{lw * FOOT};
   constexpr int RF0 = Seq<N, V, ORD_FWD>::first(), RFL = Seq<N, V, ORD_FWD>::last();
   cx<T> v[E];
@@ -87,7 +88,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
   auto pos0 = [&](int r) { return Pt{bpos<N, TPL, RF0>(tid, r), yi, zi}; };
   auto posL = [&](int r) { return Pt{bpos<N, TPL, RFL>(tid, r), yi, zi}; };
 
-  if constexpr (MODE == RM_INV) {
+  if constexpr (MODE == RM_INV || MODE == RM_INV_FWD) {
     if (active) {
 #pragma unroll
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(tid, r)];
@@ -113,7 +114,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
       if (flags & F_HALFV_PRE) halfV_all<T, E>(p, v, pos0);
     }
     fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
-    if constexpr (MODE == RM_GAUGE) {
+    if constexpr (MODE == RM_GAUGE || MODE == RM_GAUGE_FWD) {
       gauge_all<T, E, 0>(p, v, posL, a.invN);
       fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
     } else if constexpr (MODE == RM_FULL) {
@@ -136,8 +137,14 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
     }
   }
 
-  // store: RM_FWD leaves the forward layout (last radix), the others the first radix
-  if constexpr (MODE == RM_FWD) {
+  // bridge modes continue with the next step's first row sweep
+  if constexpr (MODE == RM_INV_FWD || MODE == RM_GAUGE_FWD) {
+    if (flags & F_HALFV_PRE) halfV_all<T, E>(p, v, pos0);
+    fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
+  }
+
+  // store: forward sweeps leave the forward layout (last radix), the others the first radix
+  if constexpr (MODE == RM_FWD || MODE == RM_INV_FWD || MODE == RM_GAUGE_FWD) {
     if (active) {
 #pragma unroll
       for (int r = 0; r < E; ++r) out[bpos<N, TPL, RFL>(tid, r)] = v[r];
@@ -179,7 +186,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
                                   : (long)(active ? xi : 0) + (long)o * p.xDim;
   const cx<T>* __restrict__ in = (const cx<T>*)a.in + base;
   cx<T>* __restrict__ out = (cx<T>*)a.out + base;
-  const cx<T>* tw = twiddle_source<T, N, P::NP>(a, ColCfg<T, N, V>::WG);
+  const auto* tw = twiddle_source<T, N, P::NP>(a, ColCfg<T, N, V>::WG);
   const LdsCol la{c, TC};
   constexpr int RF0 = Seq<N, V, ORD_FWD>::first(), RFL = Seq<N, V, ORD_FWD>::last();
   cx<T> v[E];

@@ -13,7 +13,9 @@ inline int col_blocks(int xDim, int others) {
 
 // dynamic LDS of a launch: the twiddle copy when enabled and the plan has twiddles
 template <typename T, int N, int V>
-inline size_t tw_shmem(int tw_lds) { return tw_lds && Plan<N, V>::NP > 1 ? (size_t)N * sizeof(cx<T>) : 0; }
+inline size_t tw_shmem(int tw_lds) {
+  return tw_lds && Plan<N, V>::NP > 1 ? (size_t)N * sizeof(typename TwE<T>::type) : 0;
+}
 
 template <typename T, int N, int V>
 hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
@@ -25,6 +27,8 @@ hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
     case RM_GAUGE: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE>), grid, block, sh, s, a); break;
     case RM_FULL: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FULL>), grid, block, sh, s, a); break;
     case RM_KSUM: hipLaunchKernelGGL((row_sweep<T, N, V, RM_KSUM>), grid, block, sh, s, a); break;
+    case RM_INV_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD>), grid, block, sh, s, a); break;
+    case RM_GAUGE_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE_FWD>), grid, block, sh, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -25,7 +25,9 @@ enum { A_ROTATION = 0, A_ZERO = 1, A_TABLE = 2 };
 enum { K_SEP = 0, K_TABLE = 1 };
 
 // Sweep modes: row (x) and strided (y/z) kernels.
-enum { RM_FWD = 0, RM_INV = 1, RM_GAUGE = 2, RM_FULL = 3, RM_KSUM = 4 };
+// RM_INV_FWD / RM_GAUGE_FWD: the last row sweep of a real-time step fused with
+// the first (RM_FWD) of the next one: ..., [V/2 of step i+1 (F_HALFV_PRE)], FFT.
+enum { RM_FWD = 0, RM_INV = 1, RM_GAUGE = 2, RM_FULL = 3, RM_KSUM = 4, RM_INV_FWD = 5, RM_GAUGE_FWD = 6 };
 enum { CM_FWD = 0, CM_INV = 1, CM_K = 2, CM_GAUGE = 3, CM_KSUM = 4 };
 
 // Flags of a sweep (runtime-uniform).
@@ -254,30 +256,35 @@ struct Pt {
   int x, y, z;
 };
 
+// The operator arithmetic runs in the working precision T: FP64 sweeps in
+// double; FP32 sweeps in float, with the table values (held in double) rounded
+// once at load.  Only the reductions stay in FP64 in the FP32 form.
+
 // psi * exp(-arg) (imaginary time) or psi * exp(-i arg) (real time), times s
 template <typename T, int E>
-__device__ __forceinline__ void apply_phase_all(cx<T>* v, const double (&arg)[E], int gstate, double s) {
+__device__ __forceinline__ void apply_phase_all(cx<T>* v, const T (&arg)[E], int gstate, double s_) {
+  const T s = (T)s_;
   if (gstate == 0) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-      const T f = WMath<T>::ex((T)-arg[r]) * (T)s;
+      const T f = WMath<T>::ex(-arg[r]) * s;
       v[r] = cscale(v[r], f);
     }
   } else {
     // one range check for the lane's E arguments, then branch-free cores
     bool fast = true;
 #pragma unroll
-    for (int r = 0; r < E; ++r) fast = fast && WMath<T>::sc_ok((T)-arg[r]);
+    for (int r = 0; r < E; ++r) fast = fast && WMath<T>::sc_ok(-arg[r]);
     T sn[E], cs[E];
     if (fast) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) WMath<T>::sc_core((T)-arg[r], &sn[r], &cs[r]);
+      for (int r = 0; r < E; ++r) WMath<T>::sc_core(-arg[r], &sn[r], &cs[r]);
     } else {
 #pragma unroll
-      for (int r = 0; r < E; ++r) WMath<T>::sc((T)-arg[r], &sn[r], &cs[r]);
+      for (int r = 0; r < E; ++r) WMath<T>::sc(-arg[r], &sn[r], &cs[r]);
     }
 #pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = cmul(mk((T)(cs[r] * (T)s), (T)(sn[r] * (T)s)), v[r]);
+    for (int r = 0; r < E; ++r) v[r] = cmul(mk(cs[r] * s, sn[r] * s), v[r]);
   }
 }
 
@@ -285,27 +292,29 @@ __device__ __forceinline__ void apply_phase_all(cx<T>* v, const double (&arg)[E]
 // with GV/EV (src/operators.cu:1081,1100) merged into one exponential.
 template <typename T, int E, typename F>
 __device__ __forceinline__ void halfV_all(const Phys& p, cx<T>* v, F pos) {
-  double arg[E];
+  T arg[E];
   if (p.v_kind == V_SEP) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const Pt q = pos(r);
-      arg[r] = p.vs[0][q.x] + p.vs[1][q.y] + p.vs[2][q.z];
+      arg[r] = (T)(p.vs[0][q.x] + p.vs[1][q.y] + p.vs[2][q.z]);
     }
   } else {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const Pt q = pos(r);
-      arg[r] = p.Vt[gid_of(p, q.x, q.y, q.z)];
+      arg[r] = (T)p.Vt[gid_of(p, q.x, q.y, q.z)];
     }
   }
+  const T cV = (T)p.cV;
 #pragma unroll
-  for (int r = 0; r < E; ++r) arg[r] = arg[r] * p.cV;
+  for (int r = 0; r < E; ++r) arg[r] = arg[r] * cV;
   if (p.gpe) {
+    const T g = (T)p.g, cH = (T)p.cH;
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-      const double d = (double)v[r].x * (double)v[r].x + (double)v[r].y * (double)v[r].y;
-      arg[r] += p.g * d * p.cH;
+      const T d = v[r].x * v[r].x + v[r].y * v[r].y;
+      arg[r] += g * d * cH;
     }
   }
   apply_phase_all<T, E>(v, arg, p.gstate, 1.0);
@@ -315,12 +324,12 @@ __device__ __forceinline__ void halfV_all(const Phys& p, cx<T>* v, F pos) {
 // 1/sqrt(n) of the axis transforms folded into invn.
 template <typename T, int E, int AXIS, typename F>
 __device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double invn) {
-  double arg[E];
+  T arg[E];
   if (p.a_kind == A_TABLE) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const Pt q = pos(r);
-      arg[r] = p.At[AXIS][gid_of(p, q.x, q.y, q.z)];
+      arg[r] = (T)p.At[AXIS][gid_of(p, q.x, q.y, q.z)];
     }
   } else {
 #pragma unroll
@@ -331,14 +340,15 @@ __device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double
         if constexpr (AXIS == 0) A = -p.yg[q.y] * p.omega * p.omegaX;
         if constexpr (AXIS == 1) A = p.xg[q.x] * p.omega * p.omegaY;
       }
-      arg[r] = A;
+      arg[r] = (T)A;
     }
   }
+  const T Dt = (T)p.Dt;
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const Pt q = pos(r);
-    const double k = AXIS == 0 ? p.pxg[q.x] : AXIS == 1 ? p.pyg[q.y] : p.pzg[q.z];
-    arg[r] = arg[r] * k * p.Dt;
+    const T k = (T)(AXIS == 0 ? p.pxg[q.x] : AXIS == 1 ? p.pyg[q.y] : p.pzg[q.z]);
+    arg[r] = arg[r] * k * Dt;
   }
   apply_phase_all<T, E>(v, arg, p.gstate, invn);
 }
@@ -347,74 +357,78 @@ __device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double
 // src/evolution.cu:761,777 folded into invN.  For the separable K the operator
 // is the product of per-axis factors (no transcendental per point).
 template <typename T, int E, typename F>
-__device__ __forceinline__ void kstep_all(const Phys& p, cx<T>* v, F pos, double invN) {
+__device__ __forceinline__ void kstep_all(const Phys& p, cx<T>* v, F pos, double invN_) {
+  const T invN = (T)invN_;
   if (p.k_kind == K_SEP) {
-    double f[E][6];
+    T f[E][6];
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const Pt q = pos(r);
       const double* fx = p.kf[0] + 2 * q.x;
       const double* fy = p.kf[1] + 2 * q.y;
       const double* fz = p.kf[2] + 2 * q.z;
-      f[r][0] = fx[0]; f[r][1] = fx[1]; f[r][2] = fy[0]; f[r][3] = fy[1]; f[r][4] = fz[0]; f[r][5] = fz[1];
+      f[r][0] = (T)fx[0]; f[r][1] = (T)fx[1]; f[r][2] = (T)fy[0];
+      f[r][3] = (T)fy[1]; f[r][4] = (T)fz[0]; f[r][5] = (T)fz[1];
     }
     if (p.gstate == 0) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = cscale(v[r], (T)(f[r][0] * f[r][2] * f[r][4] * invN));
+      for (int r = 0; r < E; ++r) v[r] = cscale(v[r], f[r][0] * f[r][2] * f[r][4] * invN);
     } else {
 #pragma unroll
       for (int r = 0; r < E; ++r) {
-        const double ar = f[r][0] * f[r][2] - f[r][1] * f[r][3], ai = f[r][0] * f[r][3] + f[r][1] * f[r][2];
-        const double br = (ar * f[r][4] - ai * f[r][5]) * invN, bi = (ar * f[r][5] + ai * f[r][4]) * invN;
-        v[r] = cmul(mk((T)br, (T)bi), v[r]);
+        const T ar = f[r][0] * f[r][2] - f[r][1] * f[r][3], ai = f[r][0] * f[r][3] + f[r][1] * f[r][2];
+        const T br = (ar * f[r][4] - ai * f[r][5]) * invN, bi = (ar * f[r][5] + ai * f[r][4]) * invN;
+        v[r] = cmul(mk(br, bi), v[r]);
       }
     }
     return;
   }
-  double arg[E];
+  T arg[E];
+  const T cK = (T)p.cK;
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const Pt q = pos(r);
-    arg[r] = p.Kt[gid_of(p, q.x, q.y, q.z)] * p.cK;
+    arg[r] = (T)p.Kt[gid_of(p, q.x, q.y, q.z)] * cK;
   }
-  apply_phase_all<T, E>(v, arg, p.gstate, invN);
+  apply_phase_all<T, E>(v, arg, p.gstate, invN_);
 }
 
 // K step on a strided line (axis AXIS = 1 or 2 varies along the line; x = xi
 // and the other coordinate o are fixed): the fixed axes' factors are loaded
 // once, only the varying axis' factor per point.  Same products as kstep_all.
 template <typename T, int E, int AXIS, typename F>
-__device__ __forceinline__ void kstep_col(const Phys& p, cx<T>* v, int xi, int o, F epos, double invN) {
+__device__ __forceinline__ void kstep_col(const Phys& p, cx<T>* v, int xi, int o, F epos, double invN_) {
   if (p.k_kind != K_SEP) {
     kstep_all<T, E>(p, v, [&](int r) { const int e = epos(r); return AXIS == 1 ? Pt{xi, e, o} : Pt{xi, o, e}; },
-                    invN);
+                    invN_);
     return;
   }
-  const double fx0 = p.kf[0][2 * xi], fx1 = p.kf[0][2 * xi + 1];
+  const T invN = (T)invN_;
+  const T fx0 = (T)p.kf[0][2 * xi], fx1 = (T)p.kf[0][2 * xi + 1];
   const double* fop = p.kf[AXIS == 1 ? 2 : 1] + 2 * o;
-  const double fo0 = fop[0], fo1 = fop[1];
+  const T fo0 = (T)fop[0], fo1 = (T)fop[1];
   const double* fvp = p.kf[AXIS];
-  double fe[E][2];
+  T fe[E][2];
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int e = epos(r);
-    fe[r][0] = fvp[2 * e];
-    fe[r][1] = fvp[2 * e + 1];
+    fe[r][0] = (T)fvp[2 * e];
+    fe[r][1] = (T)fvp[2 * e + 1];
   }
   if (p.gstate == 0) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-      const double fy0 = AXIS == 1 ? fe[r][0] : fo0, fz0 = AXIS == 1 ? fo0 : fe[r][0];
-      v[r] = cscale(v[r], (T)(fx0 * fy0 * fz0 * invN));
+      const T fy0 = AXIS == 1 ? fe[r][0] : fo0, fz0 = AXIS == 1 ? fo0 : fe[r][0];
+      v[r] = cscale(v[r], fx0 * fy0 * fz0 * invN);
     }
   } else {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-      const double fy0 = AXIS == 1 ? fe[r][0] : fo0, fy1 = AXIS == 1 ? fe[r][1] : fo1;
-      const double fz0 = AXIS == 1 ? fo0 : fe[r][0], fz1 = AXIS == 1 ? fo1 : fe[r][1];
-      const double ar = fx0 * fy0 - fx1 * fy1, ai = fx0 * fy1 + fx1 * fy0;
-      const double br = (ar * fz0 - ai * fz1) * invN, bi = (ar * fz1 + ai * fz0) * invN;
-      v[r] = cmul(mk((T)br, (T)bi), v[r]);
+      const T fy0 = AXIS == 1 ? fe[r][0] : fo0, fy1 = AXIS == 1 ? fe[r][1] : fo1;
+      const T fz0 = AXIS == 1 ? fo0 : fe[r][0], fz1 = AXIS == 1 ? fo1 : fe[r][1];
+      const T ar = fx0 * fy0 - fx1 * fy1, ai = fx0 * fy1 + fx1 * fy0;
+      const T br = (ar * fz0 - ai * fz1) * invN, bi = (ar * fz1 + ai * fz0) * invN;
+      v[r] = cmul(mk(br, bi), v[r]);
     }
   }
 }

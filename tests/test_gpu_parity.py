@@ -199,17 +199,27 @@ def test_real_time_after_imaginary_time(graph):
     ctx.close()
 
 
+GRAPH_CASES = {
+    "2d-gauge": dict(dimnum=2, xDim=128, yDim=128, xMax=10, yMax=10, g=200, gpe=1, corotating=1, omega=0.7),
+    "2d-plain": dict(dimnum=2, xDim=128, yDim=128, xMax=10, yMax=10, g=200, gpe=1),
+    "3d-plain": dict(dimnum=3, xDim=32, yDim=32, zDim=32, xMax=6, yMax=6, zMax=6, g=100, gpe=1, omegaZ=2.0),
+    "3d-gauge": dict(dimnum=3, xDim=32, yDim=32, zDim=32, xMax=6, yMax=6, zMax=6, g=100, gpe=1, omegaZ=2.0,
+                     corotating=1, omega=0.6),
+}
+
+
+@pytest.mark.parametrize("case", sorted(GRAPH_CASES))
 @pytest.mark.parametrize("gstate", [0, 1])
-def test_graph_replay_matches_eager(gstate):
-    # steady-state steps replayed from a captured HIP graph must give exactly the
-    # eagerly launched result; replays are interleaved with eager work and syncs
-    kw = dict(dimnum=2, xDim=128, yDim=128, xMax=10, yMax=10, g=200, gpe=1, corotating=1, omega=0.7, gdt=1e-3,
-              dt=5e-4)
+def test_graph_replay_matches_eager(gstate, case):
+    # steady-state steps replayed from a captured HIP graph (in real time with
+    # the row sweeps at step boundaries fused) must give exactly the eagerly
+    # launched result; replays are interleaved with eager work and syncs
+    kw = dict(GRAPH_CASES[case], gdt=1e-3, dt=5e-4)
     a = G.Context(**kw)
     b = G.Context(**kw)
     a.set_graph_steps(8)
     b.set_graph_steps(0)
-    psi = seeded_psi(a.shape, [a.grid(0), a.grid(1)], seed=5, phase=True)
+    psi = seeded_psi(a.shape, [a.grid(ax) for ax in range(a.dimnum)], seed=5, phase=True)
     for c in (a, b):
         c.upload(psi)
         c.parsum()
