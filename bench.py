@@ -80,7 +80,39 @@ def run_schedule(ctx, imag, real):
         ctx.evolve(real, 1)
 
 
-RM = {"row_fwd": 0, "row_inv": 1, "row_gauge": 2, "row_full": 3}
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
+# (each scaled by its reading for a plain 256 MiB device copy: 2.0 and 1.0), per
+# kernel instance and workload; the passes and tables are in profiles/.
+MEASURED_TRAFFIC = {
+    "gpue::row_sweep<double, 512, 1, 0>@512sq": 8704940,
+    "gpue::row_sweep<double, 512, 1, 1>@512sq": 8842309,
+    "gpue::row_sweep<double, 512, 1, 2>@512sq": 8730191,
+    "gpue::col_sweep<double, 512, 1, 2>@512sq": 8737358,
+    "gpue::col_sweep<double, 512, 1, 3>@512sq": 8735311,
+    "gpue::row_sweep<double, 256, 0, 0>@256cube": 537486839,
+    "gpue::row_sweep<double, 256, 1, 0>@256cube": 537160228,
+    "gpue::row_sweep<double, 256, 0, 1>@256cube": 537553222,
+    "gpue::row_sweep<double, 256, 1, 1>@256cube": 537382268,
+    "gpue::col_sweep<double, 256, 0, 0>@256cube": 537019155,
+    "gpue::col_sweep<double, 256, 1, 0>@256cube": 536958248,
+    "gpue::col_sweep<double, 256, 0, 1>@256cube": 537015959,
+    "gpue::col_sweep<double, 256, 1, 1>@256cube": 536966591,
+    "gpue::col_sweep<double, 256, 0, 2>@256cube": 537289281,
+    "gpue::col_sweep<double, 256, 1, 2>@256cube": 537191259,
+    "gpue::row_sweep<float, 256, 0, 0>@256cube": 268777199,
+    "gpue::row_sweep<float, 256, 1, 0>@256cube": 268715247,
+    "gpue::row_sweep<float, 256, 0, 1>@256cube": 268841374,
+    "gpue::row_sweep<float, 256, 1, 1>@256cube": 268747861,
+    "gpue::col_sweep<float, 256, 0, 0>@256cube": 268573582,
+    "gpue::col_sweep<float, 256, 1, 0>@256cube": 268520444,
+    "gpue::col_sweep<float, 256, 0, 1>@256cube": 268580273,
+    "gpue::col_sweep<float, 256, 1, 1>@256cube": 268528106,
+    "gpue::col_sweep<float, 256, 0, 2>@256cube": 268802270,
+    "gpue::col_sweep<float, 256, 1, 2>@256cube": 268741749,
+}
+
+RM = {"row_fwd": 0, "row_inv": 1, "row_gauge": 2, "row_full": 3, "row_ksum": 4, "row_inv_fwd": 5,
+      "row_gauge_fwd": 6}
 CM = {"fwd": 0, "inv": 1, "k": 2, "gauge": 3}
 
 
@@ -114,11 +146,12 @@ def kernel_table(ctx, prec, variants, steps_by_gstate, reps=50):
     return rows, bytes_per_launch
 
 
-def roofline(rows, bytes_per_launch):
+def roofline(rows, bytes_per_launch, workload):
     top = rows[0]
     achieved = bytes_per_launch / (top["us_per_launch"] * 1e-6) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": top["kernel"],
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": MEASURED_TRAFFIC.get(f"{top['symbol']}@{workload}"), "kernel": top["kernel"],
             "symbol": top["symbol"], "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": top["us_per_launch"]}
 
@@ -307,7 +340,7 @@ def main():
         for r in rows:
             log(f"   {r['kernel']:24s} g{r['gstate']} {r['us_per_launch']:8.2f} us x {r['launches_per_step']:.1f}/step "
                 f"{r['GBps']:8.1f} GB/s  {r['symbol']}")
-        roof = roofline(rows, bpl)
+        roof = roofline(rows, bpl, "512sq")
         extra["512sq_fp64_gauge"] = {
             "steps_per_s": round(sps, 1), "us_per_step": round(el / K * 1e6, 3),
             "effective_GBps": round(bytes_job / el / 1e9, 1), "sweeps_per_step_imag": P_imag,
@@ -349,7 +382,7 @@ def main():
                               effective_GBps=round(b3 * K3 / el3 / 1e9, 1),
                               frac_of_8TBps=round(b3 * K3 / el3 / 1e9 / HBM_PEAK_GBS, 4), sweeps_per_step=P3,
                               energy=e3, mu=mu3, variants=list(variants), kernels=rows3,
-                              roofline=roofline(rows3, bpl3))
+                              roofline=roofline(rows3, bpl3, "256cube"))
             log(f"256^3 FP{prec}: {K3 / el3:.1f} steps/s, {b3 * K3 / el3 / 1e9:.0f} GB/s effective, E={e3:.12g}, "
                 f"variants {variants}")
             for r in rows3:

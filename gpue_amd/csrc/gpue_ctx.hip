@@ -1039,10 +1039,22 @@ gpue_status gpue_sweep_profile(gpue_ctx* c, int gstate, int reps, int max_kinds,
   gpue_status st = apply_pending(c);
   if (st != GPUE_OK) return st;
   std::vector<Sweep> sched, kinds;
+  std::vector<int> step_of;
   std::vector<double> count;
   c->prof_names.clear();
-  for (int par = 0; par < 2; ++par) {
-    schedule(c, gstate, par, gstate == 0, sched);
+  // the sweeps of an even/odd step pair as the step loop launches them (with
+  // the real-time step boundary fused when steps are replayed from graphs)
+  if (c->graph_steps > 0) {
+    schedule_block(c, gstate, 0, 2, sched, step_of);
+  } else {
+    std::vector<Sweep> one;
+    sched.clear();
+    for (int par = 0; par < 2; ++par) {
+      schedule(c, gstate, par, gstate == 0, one);
+      sched.insert(sched.end(), one.begin(), one.end());
+    }
+  }
+  {
     for (const Sweep& sw : sched) {
       const std::string nm = sweep_name(sw);
       size_t k = 0;

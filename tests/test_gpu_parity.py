@@ -211,9 +211,13 @@ GRAPH_CASES = {
 @pytest.mark.parametrize("case", sorted(GRAPH_CASES))
 @pytest.mark.parametrize("gstate", [0, 1])
 def test_graph_replay_matches_eager(gstate, case):
-    # steady-state steps replayed from a captured HIP graph (in real time with
-    # the row sweeps at step boundaries fused) must give exactly the eagerly
-    # launched result; replays are interleaved with eager work and syncs
+    # steady-state steps replayed from a captured HIP graph must give the
+    # eagerly launched result; replays are interleaved with eager work and
+    # syncs.  Imaginary time: bit-exact.  Real time: the graph fuses the row
+    # sweeps at step boundaries into one kernel, whose code the compiler may
+    # contract into FMAs differently from the two separate kernels, so the bar
+    # is 1e-13 relative (measured: ~1e-15 after 8 steps, both paths ~1e-15
+    # from the oracle).
     kw = dict(GRAPH_CASES[case], gdt=1e-3, dt=5e-4)
     a = G.Context(**kw)
     b = G.Context(**kw)
@@ -229,8 +233,13 @@ def test_graph_replay_matches_eager(gstate, case):
         rb, _ = b.evolve(17, gstate, first_step=step)
         step += 17
         wa, wb = a.download(), b.download()
-        assert np.array_equal(wa, wb), rep
-        assert a.energy() == b.energy()
+        if gstate == 0:
+            assert np.array_equal(wa, wb), rep
+            assert a.energy() == b.energy()
+        else:
+            assert rel_l2(wa, wb) <= 1e-13, rep
+            (ea, ma), (eb, mb) = a.energy(), b.energy()
+            assert abs(ea - eb) <= 1e-13 * abs(eb) and abs(ma - mb) <= 1e-13 * abs(mb)
     a.close()
     b.close()
 
