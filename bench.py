@@ -204,6 +204,7 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
     B = G.Batch(ctxs)
     active = [True] * n_inst
     e_prev = [None] * n_inst
+    last_change = [None] * n_inst
     steps = [0] * n_inst
     point_steps = 0.0
     bytes_total = 0.0
@@ -219,6 +220,8 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
             point_steps += 100 * c.N
             bytes_total += 100 * per_step_bytes[k]
             e, _ = c.energy()
+            if e_prev[k] is not None:
+                last_change[k] = abs(e - e_prev[k]) / abs(e_prev[k])
             if e_prev[k] is not None and abs(e - e_prev[k]) < tol * abs(e_prev[k]):
                 active[k] = False
                 done_at[k] = time.perf_counter() - t0
@@ -235,7 +238,10 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
             "grid_point_steps_per_s": round(point_steps / el, 1),
             "effective_GBps": round(bytes_total / el / 1e9, 1),
             "variants_by_size": {int(k): list(v) for k, v in tuned.items()},
-            "steps_to_converge": {"min": int(min(steps)), "max": int(max(steps))}}
+            "steps_to_converge": {"min": int(min(steps)), "max": int(max(steps))},
+            "last_rel_energy_change_per_100_steps": {
+                "median": float(np.median([x for x in last_change if x is not None])),
+                "max": float(max(x for x in last_change if x is not None))}}
 
 
 def serving_stream(G, n_req=48, rate=200.0, seed=5, budget=250e9, slots=8):

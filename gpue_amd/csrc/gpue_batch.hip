@@ -109,7 +109,6 @@ static void fill_args(gpue_batch* b) {
               a.axis = sw.axis;
               a.gauge_axis = sw.axis;
               a.flags = sw.flags;
-              a.tw_lds = (c->var[sw.axis] >> 1) & 1;
               a.invN = sw.invN;
               a.norm_in = c->results;
               a.partials = c->partials;

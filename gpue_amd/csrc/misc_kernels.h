@@ -39,7 +39,7 @@ struct FinArgs {
 hipError_t launch_finalize_batched(const FinArgs* d, int ninst, hipStream_t s);
 hipError_t launch_set_double(double* p, double v, hipStream_t s);
 hipError_t launch_math_eval(int fn, const double* x, double* y, long n, hipStream_t s);
-constexpr int N_VARIANTS = 4;   // bit 0: narrow lines, bit 1: twiddles in LDS
+constexpr int N_VARIANTS = 2;   // plan: 0 wide lines, 1 narrow lines
 bool sweep_supported(int N);
 
 }  // namespace gpue

@@ -6,8 +6,7 @@ namespace gpue {
 
 #define GPUE_DECL1(NAME_)                                                                         \
   hipError_t NAME_(bool row, int mode, const SweepArgs& a, hipStream_t s);                        \
-  hipError_t NAME_##_b(bool row, int mode, const SweepArgs* d, int ninst, int lx, int others, int tw_lds, \
-                       hipStream_t s);
+  hipError_t NAME_##_b(bool row, int mode, const SweepArgs* d, int ninst, int lx, int others, hipStream_t s);
 #define GPUE_DECL(N_)                                                                     \
   GPUE_DECL1(sweep_f64_##N_##_v0) GPUE_DECL1(sweep_f32_##N_##_v0)                         \
   GPUE_DECL1(sweep_f64_##N_##_v1) GPUE_DECL1(sweep_f32_##N_##_v1)
@@ -23,13 +22,10 @@ bool sweep_supported(int N) {
   }
 }
 
-// Kernel variant: bit 0 selects the plan (0 wide lines, 1 narrow lines), bit 1
-// the twiddle source (0 global table, 1 a copy in LDS).
-hipError_t launch_sweep(int prec, bool row, int N, int var, int mode, const SweepArgs& a0, hipStream_t s,
+// Kernel variant: the plan (0 wide lines, 1 narrow lines).
+hipError_t launch_sweep(int prec, bool row, int N, int var, int mode, const SweepArgs& a, hipStream_t s,
                         bool* handled) {
   *handled = true;
-  SweepArgs a = a0;
-  a.tw_lds = (var >> 1) & 1;
   const int pv = var & 1;
 #define GPUE_CASE(N_)                                                                         \
   case N_:                                                                                    \
@@ -46,14 +42,14 @@ hipError_t launch_sweep(int prec, bool row, int N, int var, int mode, const Swee
 hipError_t launch_sweep_batched(int prec, bool row, int N, int var, int mode, const SweepArgs* d, int ninst, int lx,
                                 int others, hipStream_t s, bool* handled) {
   *handled = true;
-  const int pv = var & 1, tl = (var >> 1) & 1;   // the per-instance arguments carry tw_lds = tl
+  const int pv = var & 1;
 #define GPUE_CASE(N_)                                                                                  \
   case N_:                                                                                             \
     if (pv == 0)                                                                                       \
-      return prec == 64 ? sweep_f64_##N_##_v0_b(row, mode, d, ninst, lx, others, tl, s)                \
-                        : sweep_f32_##N_##_v0_b(row, mode, d, ninst, lx, others, tl, s);               \
-    return prec == 64 ? sweep_f64_##N_##_v1_b(row, mode, d, ninst, lx, others, tl, s)                  \
-                      : sweep_f32_##N_##_v1_b(row, mode, d, ninst, lx, others, tl, s);
+      return prec == 64 ? sweep_f64_##N_##_v0_b(row, mode, d, ninst, lx, others, s)                \
+                        : sweep_f32_##N_##_v0_b(row, mode, d, ninst, lx, others, s);               \
+    return prec == 64 ? sweep_f64_##N_##_v1_b(row, mode, d, ninst, lx, others, s)                  \
+                      : sweep_f32_##N_##_v1_b(row, mode, d, ninst, lx, others, s);
   switch (N) {
     GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
     GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)

@@ -29,26 +29,6 @@ struct ColCfg {
   static constexpr int WG = TPL * TC;
 };
 
-// Twiddle table of a sweep: the global table, or (tw_lds) a copy in the
-// launch's dynamic LDS.  The copy takes the table's loads off the critical
-// path of each pass (they are issued once, with the data loads); it costs
-// N*sizeof(cx<T>) of LDS per workgroup, so the autotuner decides per axis.
-// The first read of the table follows the first exchange barrier.
-template <typename T, int N, int NP>
-__device__ __forceinline__ const typename TwE<T>::type* twiddle_source(const SweepArgs& a, int wg) {
-  using W = typename TwE<T>::type;
-  const W* tw = (const W*)a.tw;
-  if constexpr (NP > 1) {
-    if (a.tw_lds) {
-      extern __shared__ __align__(16) unsigned char gpue_dyn_lds[];
-      W* t = reinterpret_cast<W*>(gpue_dyn_lds);
-      for (int m = threadIdx.x; m < N; m += wg) t[m] = tw[m];
-      return t;
-    }
-  }
-  return tw;
-}
-
 // scalarDiv_wfcNorm (src/kernels.cu:479-486) divides by sqrt(sum * dV); the
 // fused form multiplies by its reciprocal (within 1 ulp of the quotient).
 template <typename T>
@@ -79,7 +59,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
   const int yi = active ? line % p.yDim : 0, zi = active ? line / p.yDim : 0;
   const cx<T>* __restrict__ in = (const cx<T>*)a.in + (long)(active ? line : 0) * N;
   cx<T>* __restrict__ out = (cx<T>*)a.out + (long)(active ? line : 0) * N;
-  const auto* tw = twiddle_source<T, N, P::NP>(a, RowCfg<N, V>::WG);
+  const auto* __restrict__ tw = (const typename TwE<T>::type*)a.tw;
   const LdsRow la{lw * FOOT};
   constexpr int RF0 = Seq<N, V, ORD_FWD>::first(), RFL = Seq<N, V, ORD_FWD>::last();
   cx<T> v[E];
@@ -186,7 +166,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
                                   : (long)(active ? xi : 0) + (long)o * p.xDim;
   const cx<T>* __restrict__ in = (const cx<T>*)a.in + base;
   cx<T>* __restrict__ out = (cx<T>*)a.out + base;
-  const auto* tw = twiddle_source<T, N, P::NP>(a, ColCfg<T, N, V>::WG);
+  const auto* __restrict__ tw = (const typename TwE<T>::type*)a.tw;
   const LdsCol la{c, TC};
   constexpr int RF0 = Seq<N, V, ORD_FWD>::first(), RFL = Seq<N, V, ORD_FWD>::last();
   cx<T> v[E];

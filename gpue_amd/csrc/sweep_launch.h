@@ -11,24 +11,17 @@ inline int col_blocks(int xDim, int others) {
   return ((xDim + ColCfg<T, N, V>::TC - 1) / ColCfg<T, N, V>::TC) * others;
 }
 
-// dynamic LDS of a launch: the twiddle copy when enabled and the plan has twiddles
-template <typename T, int N, int V>
-inline size_t tw_shmem(int tw_lds) {
-  return tw_lds && Plan<N, V>::NP > 1 ? (size_t)N * sizeof(typename TwE<T>::type) : 0;
-}
-
 template <typename T, int N, int V>
 hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
   dim3 grid(row_blocks<N, V>(a.lines)), block(RowCfg<N, V>::WG);
-  const size_t sh = tw_shmem<T, N, V>(a.tw_lds);
   switch (mode) {
-    case RM_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FWD>), grid, block, sh, s, a); break;
-    case RM_INV: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV>), grid, block, sh, s, a); break;
-    case RM_GAUGE: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE>), grid, block, sh, s, a); break;
-    case RM_FULL: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FULL>), grid, block, sh, s, a); break;
-    case RM_KSUM: hipLaunchKernelGGL((row_sweep<T, N, V, RM_KSUM>), grid, block, sh, s, a); break;
-    case RM_INV_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD>), grid, block, sh, s, a); break;
-    case RM_GAUGE_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE_FWD>), grid, block, sh, s, a); break;
+    case RM_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FWD>), grid, block, 0, s, a); break;
+    case RM_INV: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV>), grid, block, 0, s, a); break;
+    case RM_GAUGE: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE>), grid, block, 0, s, a); break;
+    case RM_FULL: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FULL>), grid, block, 0, s, a); break;
+    case RM_KSUM: hipLaunchKernelGGL((row_sweep<T, N, V, RM_KSUM>), grid, block, 0, s, a); break;
+    case RM_INV_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD>), grid, block, 0, s, a); break;
+    case RM_GAUGE_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE_FWD>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -38,13 +31,12 @@ template <typename T, int N, int V>
 hipError_t launch_col_t(int mode, const SweepArgs& a, hipStream_t s) {
   const int others = a.axis == 1 ? a.ph.zDim : a.ph.yDim;
   dim3 grid(col_blocks<T, N, V>(a.ph.xDim, others)), block(ColCfg<T, N, V>::WG);
-  const size_t sh = tw_shmem<T, N, V>(a.tw_lds);
   switch (mode) {
-    case CM_FWD: hipLaunchKernelGGL((col_sweep<T, N, V, CM_FWD>), grid, block, sh, s, a); break;
-    case CM_INV: hipLaunchKernelGGL((col_sweep<T, N, V, CM_INV>), grid, block, sh, s, a); break;
-    case CM_K: hipLaunchKernelGGL((col_sweep<T, N, V, CM_K>), grid, block, sh, s, a); break;
-    case CM_GAUGE: hipLaunchKernelGGL((col_sweep<T, N, V, CM_GAUGE>), grid, block, sh, s, a); break;
-    case CM_KSUM: hipLaunchKernelGGL((col_sweep<T, N, V, CM_KSUM>), grid, block, sh, s, a); break;
+    case CM_FWD: hipLaunchKernelGGL((col_sweep<T, N, V, CM_FWD>), grid, block, 0, s, a); break;
+    case CM_INV: hipLaunchKernelGGL((col_sweep<T, N, V, CM_INV>), grid, block, 0, s, a); break;
+    case CM_K: hipLaunchKernelGGL((col_sweep<T, N, V, CM_K>), grid, block, 0, s, a); break;
+    case CM_GAUGE: hipLaunchKernelGGL((col_sweep<T, N, V, CM_GAUGE>), grid, block, 0, s, a); break;
+    case CM_KSUM: hipLaunchKernelGGL((col_sweep<T, N, V, CM_KSUM>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -52,30 +44,28 @@ hipError_t launch_col_t(int mode, const SweepArgs& a, hipStream_t s) {
 
 // Batched: `ninst` problems of one shape, arguments in device memory.
 template <typename T, int N, int V>
-hipError_t launch_row_b_t(int mode, const SweepArgs* d, int ninst, int lines, int tw_lds, hipStream_t s) {
+hipError_t launch_row_b_t(int mode, const SweepArgs* d, int ninst, int lines, hipStream_t s) {
   const int bpi = row_blocks<N, V>(lines);
   dim3 grid(bpi * ninst), block(RowCfg<N, V>::WG);
-  const size_t sh = tw_shmem<T, N, V>(tw_lds);
   switch (mode) {
-    case RM_FWD: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_FWD>), grid, block, sh, s, d, bpi); break;
-    case RM_INV: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_INV>), grid, block, sh, s, d, bpi); break;
-    case RM_GAUGE: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_GAUGE>), grid, block, sh, s, d, bpi); break;
-    case RM_FULL: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_FULL>), grid, block, sh, s, d, bpi); break;
+    case RM_FWD: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_FWD>), grid, block, 0, s, d, bpi); break;
+    case RM_INV: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_INV>), grid, block, 0, s, d, bpi); break;
+    case RM_GAUGE: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_GAUGE>), grid, block, 0, s, d, bpi); break;
+    case RM_FULL: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_FULL>), grid, block, 0, s, d, bpi); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 template <typename T, int N, int V>
-hipError_t launch_col_b_t(int mode, const SweepArgs* d, int ninst, int xDim, int others, int tw_lds, hipStream_t s) {
+hipError_t launch_col_b_t(int mode, const SweepArgs* d, int ninst, int xDim, int others, hipStream_t s) {
   const int bpi = col_blocks<T, N, V>(xDim, others);
   dim3 grid(bpi * ninst), block(ColCfg<T, N, V>::WG);
-  const size_t sh = tw_shmem<T, N, V>(tw_lds);
   switch (mode) {
-    case CM_FWD: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_FWD>), grid, block, sh, s, d, bpi); break;
-    case CM_INV: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_INV>), grid, block, sh, s, d, bpi); break;
-    case CM_K: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_K>), grid, block, sh, s, d, bpi); break;
-    case CM_GAUGE: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_GAUGE>), grid, block, sh, s, d, bpi); break;
+    case CM_FWD: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_FWD>), grid, block, 0, s, d, bpi); break;
+    case CM_INV: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_INV>), grid, block, 0, s, d, bpi); break;
+    case CM_K: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_K>), grid, block, 0, s, d, bpi); break;
+    case CM_GAUGE: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_GAUGE>), grid, block, 0, s, d, bpi); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -90,8 +80,8 @@ hipError_t launch_col_b_t(int mode, const SweepArgs* d, int ninst, int xDim, int
     return row ? launch_row_t<T_, N_, V_>(mode, a, s) : launch_col_t<T_, N_, V_>(mode, a, s);     \
   }                                                                                               \
   hipError_t NAME_##_b(bool row, int mode, const SweepArgs* d, int ninst, int lx, int others,     \
-                       int tw_lds, hipStream_t s) {                                               \
-    return row ? launch_row_b_t<T_, N_, V_>(mode, d, ninst, lx, tw_lds, s)                        \
-               : launch_col_b_t<T_, N_, V_>(mode, d, ninst, lx, others, tw_lds, s);               \
+                       hipStream_t s) {                                                           \
+    return row ? launch_row_b_t<T_, N_, V_>(mode, d, ninst, lx, s)                                \
+               : launch_col_b_t<T_, N_, V_>(mode, d, ninst, lx, others, s);                       \
   }                                                                                               \
   }

@@ -78,7 +78,6 @@ struct SweepArgs {
   int axis;           // 0 x (row), 1 y, 2 z
   int gauge_axis;     // axis of the A operator applied by F_GAUGE_POST / gauge modes
   unsigned flags;
-  int tw_lds;         // 1: copy the twiddle table to (dynamic) LDS first, read it there
   double invN;        // normalisation folded into K (1/N) or A (1/n_axis)
   const double* norm_in;   // pending renormalisation divisor (F_SCALE_IN)
   double* partials;        // per-workgroup partial sums (F_NORM, KSUM modes)
@@ -156,14 +155,9 @@ __device__ __forceinline__ double exp_cw(double x) {
   return ldexp(p, (int)k);
 }
 
-// core for |x| < 2^20 (no branch, so the polynomial chains of several points
-// can be interleaved); sincos_cw adds the library path for other arguments
-__device__ __forceinline__ void sincos_core(double x, double* s, double* c) {
+// sin and cos of |r| <= pi/4 (no reduction)
+__device__ __forceinline__ void sincos_poly(double r, double* sn_out, double* cs_out) {
   using namespace fm;
-  const double q = rint(x * TWO_OVER_PI);
-  double r = fma(-q, PIO2_1, x);
-  r = fma(-q, PIO2_2, r);
-  r = fma(-q, PIO2_3, r);
   const double z = r * r;
   double ps = -F17;
   ps = fma(ps, z, F15); ps = fma(ps, z, -F13); ps = fma(ps, z, F11); ps = fma(ps, z, -F9);
@@ -173,11 +167,27 @@ __device__ __forceinline__ void sincos_core(double x, double* s, double* c) {
   pc = fma(pc, z, -F16); pc = fma(pc, z, F14); pc = fma(pc, z, -F12); pc = fma(pc, z, F10);
   pc = fma(pc, z, -F8); pc = fma(pc, z, F6); pc = fma(pc, z, -F4); pc = fma(pc, z, F2);
   const double cs = fma(-z, pc, 1.0);     // 1 - r^2/2! + ...
+  *sn_out = sn;
+  *cs_out = cs;
+}
+
+// core for |x| < 2^20 (no branch, so the polynomial chains of several points
+// can be interleaved); sincos_cw adds the library path for other arguments
+__device__ __forceinline__ void sincos_core(double x, double* s, double* c) {
+  using namespace fm;
+  const double q = rint(x * TWO_OVER_PI);
+  double r = fma(-q, PIO2_1, x);
+  r = fma(-q, PIO2_2, r);
+  r = fma(-q, PIO2_3, r);
+  double sn, cs;
+  sincos_poly(r, &sn, &cs);
   const int n = ((int)q) & 3;
   const double s0 = (n & 1) ? cs : sn, c0 = (n & 1) ? sn : cs;
   *s = (n & 2) ? -s0 : s0;
   *c = ((n + 1) & 2) ? -c0 : c0;
 }
+
+__device__ __forceinline__ bool sincos_small_ok(double x) { return fabs(x) <= 0.78539816339744828; }
 
 __device__ __forceinline__ bool sincos_fast_ok(double x) { return fabs(x) < 1048576.0; }
 
@@ -193,11 +203,7 @@ __device__ __forceinline__ void sincos_cw(double x, double* s, double* c) {
 // rounded tail (q*P1, q*P2 exact for |q| < 2^12), Taylor degrees 9 and 10 on
 // |r| <= pi/4 (truncation < 2e-9, below FP32 rounding); |x| >= 6000, infinities
 // and NaN go to the library.
-__device__ __forceinline__ void sincos_core(float x, float* s, float* c) {
-  const float q = rintf(x * 0.6366197466850281f);
-  float r = fmaf(-q, 1.57080078125f, x);
-  r = fmaf(-q, -4.453584551811218e-06f, r);
-  r = fmaf(-q, -8.705515752716053e-10f, r);
+__device__ __forceinline__ void sincos_poly(float r, float* sn_out, float* cs_out) {
   const float z = r * r;
   float ps = 2.7557318844628753e-06f;
   ps = fmaf(ps, z, -0.00019841270113829523f);
@@ -210,11 +216,24 @@ __device__ __forceinline__ void sincos_core(float x, float* s, float* c) {
   pc = fmaf(pc, z, 0.0416666679084301f);
   pc = fmaf(pc, z, -0.5f);
   const float cs = fmaf(z, pc, 1.0f);
+  *sn_out = sn;
+  *cs_out = cs;
+}
+
+__device__ __forceinline__ void sincos_core(float x, float* s, float* c) {
+  const float q = rintf(x * 0.6366197466850281f);
+  float r = fmaf(-q, 1.57080078125f, x);
+  r = fmaf(-q, -4.453584551811218e-06f, r);
+  r = fmaf(-q, -8.705515752716053e-10f, r);
+  float sn, cs;
+  sincos_poly(r, &sn, &cs);
   const int n = ((int)q) & 3;
   const float s0 = (n & 1) ? cs : sn, c0 = (n & 1) ? sn : cs;
   *s = (n & 2) ? -s0 : s0;
   *c = ((n + 1) & 2) ? -c0 : c0;
 }
+
+__device__ __forceinline__ bool sincos_small_ok(float x) { return fabsf(x) <= 0.78539816f; }
 
 __device__ __forceinline__ bool sincos_fast_ok(float x) { return fabsf(x) < 6000.0f; }
 
@@ -236,6 +255,8 @@ struct WMath<double> {
   static __device__ __forceinline__ void sc(double a, double* s, double* c) { sincos_cw(a, s, c); }
   static __device__ __forceinline__ void sc_core(double a, double* s, double* c) { sincos_core(a, s, c); }
   static __device__ __forceinline__ bool sc_ok(double a) { return sincos_fast_ok(a); }
+  static __device__ __forceinline__ bool sc_small_ok(double a) { return sincos_small_ok(a); }
+  static __device__ __forceinline__ void sc_small(double a, double* s, double* c) { sincos_poly(a, s, c); }
 };
 template <>
 struct WMath<float> {
@@ -243,6 +264,8 @@ struct WMath<float> {
   static __device__ __forceinline__ void sc(float a, float* s, float* c) { sincos_cw(a, s, c); }
   static __device__ __forceinline__ void sc_core(float a, float* s, float* c) { sincos_core(a, s, c); }
   static __device__ __forceinline__ bool sc_ok(float a) { return sincos_fast_ok(a); }
+  static __device__ __forceinline__ bool sc_small_ok(float a) { return sincos_small_ok(a); }
+  static __device__ __forceinline__ void sc_small(float a, float* s, float* c) { sincos_poly(a, s, c); }
 };
 
 // ---------------------------------------------------------------------------
@@ -271,12 +294,19 @@ __device__ __forceinline__ void apply_phase_all(cx<T>* v, const T (&arg)[E], int
       v[r] = cscale(v[r], f);
     }
   } else {
-    // one range check for the lane's E arguments, then branch-free cores
-    bool fast = true;
+    // one range check for the lane's E arguments, then branch-free cores; a
+    // real-time phase is usually within pi/4 (no reduction needed)
+    bool small = true, fast = true;
 #pragma unroll
-    for (int r = 0; r < E; ++r) fast = fast && WMath<T>::sc_ok(-arg[r]);
+    for (int r = 0; r < E; ++r) {
+      small = small && WMath<T>::sc_small_ok(-arg[r]);
+      fast = fast && WMath<T>::sc_ok(-arg[r]);
+    }
     T sn[E], cs[E];
-    if (fast) {
+    if (small) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) WMath<T>::sc_small(-arg[r], &sn[r], &cs[r]);
+    } else if (fast) {
 #pragma unroll
       for (int r = 0; r < E; ++r) WMath<T>::sc_core(-arg[r], &sn[r], &cs[r]);
     } else {

@@ -131,9 +131,9 @@ gpue_status gpue_sync(gpue_ctx *c);
 /* FFT kernel shape per axis (no reference counterpart: cuFFT plans choose their
  * own).  gpue_autotune times every variant of every sweep of the current
  * configuration (graph replays of back-to-back launches on a scratch copy of
- * psi) and keeps the fastest per axis; variants[3] receives the choice, a
- * value 0..3: bit 0 selects narrow lines (fewer points per lane), bit 1 reads
- * the twiddle table from a copy in LDS.  gpue_set_variants forces a choice. */
+ * psi) and keeps the fastest per axis; variants[3] receives the choice
+ * (0 = wide lines, 1 = narrow lines: fewer points per lane).
+ * gpue_set_variants forces a choice. */
 gpue_status gpue_autotune(gpue_ctx *c, int reps, int *variants);
 gpue_status gpue_set_variants(gpue_ctx *c, const int *variants);
 /* Steady-state steps are captured as a HIP graph of `steps` consecutive steps
