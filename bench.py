@@ -117,14 +117,15 @@ CM = {"fwd": 0, "inv": 1, "k": 2, "gauge": 3}
 
 
 def kernel_symbol(name, prec, n_axis, variant):
-    """The template instance rocprofv3 reports for a sweep name (the plan is
-    bit 0 of the variant; bit 1, twiddles in LDS, is a launch argument)."""
+    """The template instance rocprofv3 reports for a sweep name (row sweeps
+    are instantiated per plan, bit 0 of the variant; strided sweeps per full
+    variant, plan and tile width)."""
     t = "double" if prec == 64 else "float"
     base = name.split("+")[0]
     if base.startswith("row"):
         return f"gpue::row_sweep<{t}, {n_axis[0]}, {variant[0] & 1}, {RM[base]}>"
     axis = 1 if base.startswith("coly") else 2
-    return f"gpue::col_sweep<{t}, {n_axis[axis]}, {variant[axis] & 1}, {CM[base.split('_', 1)[1]]}>"
+    return f"gpue::col_sweep<{t}, {n_axis[axis]}, {variant[axis]}, {CM[base.split('_', 1)[1]]}>"
 
 
 def kernel_table(ctx, prec, variants, steps_by_gstate, reps=50):

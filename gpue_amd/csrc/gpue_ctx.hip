@@ -743,6 +743,7 @@ static gpue_status autotune(gpue_ctx* c, int reps) {
       Phys ph = make_phys(c, gstate, 0.0);
       for (Sweep s : sched) {
         for (int v = 0; v < N_VARIANTS && st == GPUE_OK; ++v) {
+          if (s.row && v > 1) break;   // tile width applies to strided sweeps only
           s.var = v;
           double us = 0.0;
           st = time_sweep(c, s, ph, reps, e0, e1, &us);

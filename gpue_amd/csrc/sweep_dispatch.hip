@@ -5,8 +5,9 @@
 namespace gpue {
 
 #define GPUE_DECL1(NAME_)                                                                         \
-  hipError_t NAME_(bool row, int mode, const SweepArgs& a, hipStream_t s);                        \
-  hipError_t NAME_##_b(bool row, int mode, const SweepArgs* d, int ninst, int lx, int others, hipStream_t s);
+  hipError_t NAME_(bool row, int narrow, int mode, const SweepArgs& a, hipStream_t s);            \
+  hipError_t NAME_##_b(bool row, int narrow, int mode, const SweepArgs* d, int ninst, int lx, int others, \
+                       hipStream_t s);
 #define GPUE_DECL(N_)                                                                     \
   GPUE_DECL1(sweep_f64_##N_##_v0) GPUE_DECL1(sweep_f32_##N_##_v0)                         \
   GPUE_DECL1(sweep_f64_##N_##_v1) GPUE_DECL1(sweep_f32_##N_##_v1)
@@ -22,15 +23,17 @@ bool sweep_supported(int N) {
   }
 }
 
-// Kernel variant: the plan (0 wide lines, 1 narrow lines).
+// Kernel variant: bit 0 the plan (0 wide lines, 1 narrow lines); var >> 1
+// narrows the strided sweeps' tiles by 2^(var >> 1) (row sweeps ignore it).
 hipError_t launch_sweep(int prec, bool row, int N, int var, int mode, const SweepArgs& a, hipStream_t s,
                         bool* handled) {
   *handled = true;
-  const int pv = var & 1;
+  const int pv = var & 1, nw = var >> 1;
 #define GPUE_CASE(N_)                                                                         \
   case N_:                                                                                    \
-    if (pv == 0) return prec == 64 ? sweep_f64_##N_##_v0(row, mode, a, s) : sweep_f32_##N_##_v0(row, mode, a, s); \
-    return prec == 64 ? sweep_f64_##N_##_v1(row, mode, a, s) : sweep_f32_##N_##_v1(row, mode, a, s);
+    if (pv == 0)                                                                              \
+      return prec == 64 ? sweep_f64_##N_##_v0(row, nw, mode, a, s) : sweep_f32_##N_##_v0(row, nw, mode, a, s); \
+    return prec == 64 ? sweep_f64_##N_##_v1(row, nw, mode, a, s) : sweep_f32_##N_##_v1(row, nw, mode, a, s);
   switch (N) {
     GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
     GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
@@ -42,14 +45,14 @@ hipError_t launch_sweep(int prec, bool row, int N, int var, int mode, const Swee
 hipError_t launch_sweep_batched(int prec, bool row, int N, int var, int mode, const SweepArgs* d, int ninst, int lx,
                                 int others, hipStream_t s, bool* handled) {
   *handled = true;
-  const int pv = var & 1;
+  const int pv = var & 1, nw = var >> 1;
 #define GPUE_CASE(N_)                                                                                  \
   case N_:                                                                                             \
     if (pv == 0)                                                                                       \
-      return prec == 64 ? sweep_f64_##N_##_v0_b(row, mode, d, ninst, lx, others, s)                \
-                        : sweep_f32_##N_##_v0_b(row, mode, d, ninst, lx, others, s);               \
-    return prec == 64 ? sweep_f64_##N_##_v1_b(row, mode, d, ninst, lx, others, s)                  \
-                      : sweep_f32_##N_##_v1_b(row, mode, d, ninst, lx, others, s);
+      return prec == 64 ? sweep_f64_##N_##_v0_b(row, nw, mode, d, ninst, lx, others, s)            \
+                        : sweep_f32_##N_##_v0_b(row, nw, mode, d, ninst, lx, others, s);           \
+    return prec == 64 ? sweep_f64_##N_##_v1_b(row, nw, mode, d, ninst, lx, others, s)              \
+                      : sweep_f32_##N_##_v1_b(row, nw, mode, d, ninst, lx, others, s);
   switch (N) {
     GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
     GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
@@ -58,14 +61,20 @@ hipError_t launch_sweep_batched(int prec, bool row, int N, int var, int mode, co
 #undef GPUE_CASE
 }
 
-int sweep_blocks(int prec, bool row, int N, int var_full, int lines_or_xdim, int others) {
-  const int var = var_full & 1;
+int sweep_blocks(int prec, bool row, int N, int var, int lines_or_xdim, int others) {
 #define GPUE_COLB(N_, V_) \
   (prec == 64 ? col_blocks<double, N_, V_>(lines_or_xdim, others) : col_blocks<float, N_, V_>(lines_or_xdim, others))
 #define GPUE_CASE(N_)                                                                              \
   case N_:                                                                                         \
-    if (var == 0) return row ? row_blocks<N_, 0>(lines_or_xdim) : GPUE_COLB(N_, 0);                \
-    return row ? row_blocks<N_, 1>(lines_or_xdim) : GPUE_COLB(N_, 1);
+    if (row) return (var & 1) ? row_blocks<N_, 1>(lines_or_xdim) : row_blocks<N_, 0>(lines_or_xdim); \
+    switch (var) {                                                                                 \
+      case 0: return GPUE_COLB(N_, 0);                                                             \
+      case 1: return GPUE_COLB(N_, 1);                                                             \
+      case 2: return GPUE_COLB(N_, 2);                                                             \
+      case 3: return GPUE_COLB(N_, 3);                                                             \
+      case 4: return GPUE_COLB(N_, 4);                                                             \
+      default: return GPUE_COLB(N_, 5);                                                            \
+    }
   switch (N) {
     GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
     GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)

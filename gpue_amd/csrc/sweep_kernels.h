@@ -19,13 +19,22 @@ struct RowCfg {
 // lanes' worth for short transforms, within 1024 lanes per workgroup.
 // Measured at 256^3: FP64 8-column tiles run the strided FFT at copy speed
 // (2-4 columns lose 40-70 %); FP32 8-column tiles fetch every line twice.
-template <typename T, int N, int V>
+//
+// CV is the strided-sweep variant: bit 0 the plan, bits 1-2 narrow the tile
+// to a half or a quarter of that width. Small grids have few tiles (64 of
+// them for 512 FP64 columns), so narrower tiles put more CUs to work; the
+// tiles that share a 128-byte line then run on one XCD (col_sweep), so the
+// line is fetched into one L2 only.
+template <typename T, int N, int CV>
 struct ColCfg {
-  static constexpr int TPL = Plan<N, V>::TPL;
+  static constexpr int PV = CV & 1, NARROW = CV >> 1;
+  static constexpr int TPL = Plan<N, PV>::TPL;
   static constexpr int TCmin = 128 / (2 * (int)sizeof(T));
   static constexpr int TCa = 256 / TPL > TCmin ? 256 / TPL : TCmin;
   static constexpr int TCb = TCa > 64 ? 64 : TCa;
-  static constexpr int TC = TPL * TCb > 1024 ? 1024 / TPL : TCb;
+  static constexpr int TC0 = TPL * TCb > 1024 ? 1024 / TPL : TCb;
+  static constexpr int TC = (TC0 >> NARROW) > 0 ? (TC0 >> NARROW) : 1;
+  static constexpr int G = TC0 / TC;   // tiles per default-width tile
   static constexpr int WG = TPL * TC;
 };
 
@@ -148,10 +157,11 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 // Strided sweep: lines along y (axis 1, stride xDim) or z (axis 2, stride
 // xDim*yDim); a workgroup owns TC adjacent x-lines, lanes vary fastest in x.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int V, int MODE>
+template <typename T, int N, int CV, int MODE>
 __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) {
+  constexpr int V = CV & 1;
   using P = Plan<N, V>;
-  constexpr int TC = ColCfg<T, N, V>::TC, E = P::E, TPL = P::TPL;
+  constexpr int TC = ColCfg<T, N, CV>::TC, E = P::E, TPL = P::TPL;
   __shared__ cx<T> lds[P::NP > 1 ? TC * N : 1];
   __shared__ double sh[64];
   const Phys& p = a.ph;
@@ -240,9 +250,26 @@ template <typename T, int N, int V, int MODE>
 __global__ void __launch_bounds__((RowCfg<N, V>::WG)) row_sweep(SweepArgs a) {
   row_body<T, N, V, MODE>(a, blockIdx.x, gridDim.x);
 }
-template <typename T, int N, int V, int MODE>
-__global__ void __launch_bounds__((ColCfg<T, N, V>::WG)) col_sweep(SweepArgs a) {
-  col_body<T, N, V, MODE>(a, blockIdx.x, gridDim.x);
+// Workgroups are dealt to the 8 XCDs round robin (block b runs on XCD b % 8).
+// With narrowed tiles, the G tiles of one default-width tile (one 128-byte
+// line per row) are given blocks on the same XCD: block b = x + 8q takes
+// tile G*(x + 8*(q/G)) + q%G. Blocks past the last whole group of 8G keep
+// their own index, so the map is a permutation of [0, nblk).
+template <int G>
+__device__ __forceinline__ int xcd_tile(int b, int nblk) {
+  if constexpr (G == 1) {
+    return b;
+  } else {
+    const int full = nblk - nblk % (8 * G);
+    if (b >= full) return b;
+    const int x = b & 7, q = b >> 3;
+    return G * (x + 8 * (q / G)) + q % G;
+  }
+}
+
+template <typename T, int N, int CV, int MODE>
+__global__ void __launch_bounds__((ColCfg<T, N, CV>::WG)) col_sweep(SweepArgs a) {
+  col_body<T, N, CV, MODE>(a, xcd_tile<ColCfg<T, N, CV>::G>(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // ... or many problems of one shape per launch: blocks [i*bpi, (i+1)*bpi) work
@@ -253,11 +280,11 @@ __global__ void __launch_bounds__((RowCfg<N, V>::WG)) row_sweep_b(const SweepArg
   if (!a.active) return;
   row_body<T, N, V, MODE>(a, blockIdx.x % bpi, bpi);
 }
-template <typename T, int N, int V, int MODE>
-__global__ void __launch_bounds__((ColCfg<T, N, V>::WG)) col_sweep_b(const SweepArgs* __restrict__ args, int bpi) {
+template <typename T, int N, int CV, int MODE>
+__global__ void __launch_bounds__((ColCfg<T, N, CV>::WG)) col_sweep_b(const SweepArgs* __restrict__ args, int bpi) {
   const SweepArgs& a = args[blockIdx.x / bpi];
   if (!a.active) return;
-  col_body<T, N, V, MODE>(a, blockIdx.x % bpi, bpi);
+  col_body<T, N, CV, MODE>(a, blockIdx.x % bpi, bpi);
 }
 
 }  // namespace gpue

@@ -6,9 +6,9 @@ namespace gpue {
 
 template <int N, int V>
 inline int row_blocks(int lines) { return (lines + RowCfg<N, V>::LPW - 1) / RowCfg<N, V>::LPW; }
-template <typename T, int N, int V>
+template <typename T, int N, int CV>
 inline int col_blocks(int xDim, int others) {
-  return ((xDim + ColCfg<T, N, V>::TC - 1) / ColCfg<T, N, V>::TC) * others;
+  return ((xDim + ColCfg<T, N, CV>::TC - 1) / ColCfg<T, N, CV>::TC) * others;
 }
 
 template <typename T, int N, int V>
@@ -73,15 +73,21 @@ hipError_t launch_col_b_t(int mode, const SweepArgs* d, int ninst, int xDim, int
 
 }  // namespace gpue
 
-// One translation unit per (precision, N, variant) keeps the build parallel.
+// One translation unit per (precision, N, plan) keeps the build parallel; it
+// holds the strided sweeps of every tile width (narrow = 0, 1, 2).
 #define GPUE_SWEEP_TU(T_, N_, V_, NAME_)                                                          \
   namespace gpue {                                                                                \
-  hipError_t NAME_(bool row, int mode, const SweepArgs& a, hipStream_t s) {                       \
-    return row ? launch_row_t<T_, N_, V_>(mode, a, s) : launch_col_t<T_, N_, V_>(mode, a, s);     \
+  hipError_t NAME_(bool row, int narrow, int mode, const SweepArgs& a, hipStream_t s) {           \
+    if (row) return launch_row_t<T_, N_, V_>(mode, a, s);                                         \
+    if (narrow == 1) return launch_col_t<T_, N_, V_ + 2>(mode, a, s);                             \
+    if (narrow == 2) return launch_col_t<T_, N_, V_ + 4>(mode, a, s);                             \
+    return launch_col_t<T_, N_, V_>(mode, a, s);                                                  \
   }                                                                                               \
-  hipError_t NAME_##_b(bool row, int mode, const SweepArgs* d, int ninst, int lx, int others,     \
-                       hipStream_t s) {                                                           \
-    return row ? launch_row_b_t<T_, N_, V_>(mode, d, ninst, lx, s)                                \
-               : launch_col_b_t<T_, N_, V_>(mode, d, ninst, lx, others, s);                       \
+  hipError_t NAME_##_b(bool row, int narrow, int mode, const SweepArgs* d, int ninst, int lx,     \
+                       int others, hipStream_t s) {                                               \
+    if (row) return launch_row_b_t<T_, N_, V_>(mode, d, ninst, lx, s);                            \
+    if (narrow == 1) return launch_col_b_t<T_, N_, V_ + 2>(mode, d, ninst, lx, others, s);        \
+    if (narrow == 2) return launch_col_b_t<T_, N_, V_ + 4>(mode, d, ninst, lx, others, s);        \
+    return launch_col_b_t<T_, N_, V_>(mode, d, ninst, lx, others, s);                             \
   }                                                                                               \
   }

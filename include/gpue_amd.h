@@ -132,8 +132,10 @@ gpue_status gpue_sync(gpue_ctx *c);
  * own).  gpue_autotune times every variant of every sweep of the current
  * configuration (graph replays of back-to-back launches on a scratch copy of
  * psi) and keeps the fastest per axis; variants[3] receives the choice
- * (0 = wide lines, 1 = narrow lines: fewer points per lane).
- * gpue_set_variants forces a choice. */
+ * (0..5: bit 0 the plan, 0 = wide lines, 1 = narrow lines with fewer points
+ * per lane; variant >> 1 = 0, 1, 2 narrows the tiles of the y/z sweeps to 1,
+ * 1/2 or 1/4 of their default width, to spread small grids over more CUs;
+ * the x axis uses bit 0 only).  gpue_set_variants forces a choice. */
 gpue_status gpue_autotune(gpue_ctx *c, int reps, int *variants);
 gpue_status gpue_set_variants(gpue_ctx *c, const int *variants);
 /* Steady-state steps are captured as a HIP graph of `steps` consecutive steps

@@ -244,12 +244,12 @@ def test_graph_replay_matches_eager(gstate, case):
     b.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("name,dim,kw", [c for c in CASES if c[0] in ("1d-gauge", "2d-gauge", "3d-gauge", "2d-48",
                                                                       "3d-96")],
                          ids=["1d-gauge", "2d-gauge", "3d-gauge", "2d-48", "3d-96"])
 def test_every_kernel_variant(name, dim, kw, variant):
-    # the autotuner may pick either plan per axis: each
+    # the autotuner may pick any plan and strided tile width per axis: each
     # must meet the FP64 bar, imaginary and real time
     for gstate in (0, 1):
         ctx = G.Context(precision=64, gdt=1e-3, dt=5e-4, **kw)
