@@ -118,12 +118,12 @@ CM = {"fwd": 0, "inv": 1, "k": 2, "gauge": 3}
 
 def kernel_symbol(name, prec, n_axis, variant):
     """The template instance rocprofv3 reports for a sweep name (row sweeps
-    are instantiated per plan, bit 0 of the variant; strided sweeps per full
-    variant, plan and tile width)."""
+    are instantiated per plan and twiddle loading, variants 0..3 with 4, 5
+    acting as 2, 3; strided sweeps per full variant)."""
     t = "double" if prec == 64 else "float"
     base = name.split("+")[0]
     if base.startswith("row"):
-        return f"gpue::row_sweep<{t}, {n_axis[0]}, {variant[0] & 1}, {RM[base]}>"
+        return f"gpue::row_sweep<{t}, {n_axis[0]}, {(variant[0] & 1) + (2 if variant[0] > 1 else 0)}, {RM[base]}>"
     axis = 1 if base.startswith("coly") else 2
     return f"gpue::col_sweep<{t}, {n_axis[axis]}, {variant[axis]}, {CM[base.split('_', 1)[1]]}>"
 
@@ -333,9 +333,13 @@ def main():
         ctx.upload(psi0)
         ctx.sync()
         t0 = time.perf_counter()
-        run_schedule(ctx, imag, real)
+        ctx.evolve(imag, 0)
+        ctx.sync()   # split point for the per-phase rates (one sync in the job)
+        t_mid = time.perf_counter()
+        ctx.evolve(real, 1)
         ctx.sync()
-        el = time.perf_counter() - t0
+        t_end = time.perf_counter()
+        el = t_end - t0
         sps = K / el
         e_final, mu_final = ctx.energy()
         bps_imag, P_imag = ctx.step_bytes(0)
@@ -343,13 +347,17 @@ def main():
         bytes_job = imag * bps_imag + real * bps_real
         rows, bpl = kernel_table(ctx, 64, variants, {0: imag, 1: real})
         ctx.close()
-        log(f"512^2 FP64 gauge: {sps:.1f} steps/s ({el / K * 1e6:.2f} us/step), E={e_final:.10g}, variants {variants}")
+        log(f"512^2 FP64 gauge: {sps:.1f} steps/s ({el / K * 1e6:.2f} us/step; imaginary "
+            f"{(t_mid - t0) / max(imag, 1) * 1e6:.2f}, real {(t_end - t_mid) / max(real, 1) * 1e6:.2f}), "
+            f"E={e_final:.10g}, variants {variants}")
         for r in rows:
             log(f"   {r['kernel']:24s} g{r['gstate']} {r['us_per_launch']:8.2f} us x {r['launches_per_step']:.1f}/step "
                 f"{r['GBps']:8.1f} GB/s  {r['symbol']}")
         roof = roofline(rows, bpl, "512sq")
         extra["512sq_fp64_gauge"] = {
             "steps_per_s": round(sps, 1), "us_per_step": round(el / K * 1e6, 3),
+            "us_per_step_imag": round((t_mid - t0) / max(imag, 1) * 1e6, 3),
+            "us_per_step_real": round((t_end - t_mid) / max(real, 1) * 1e6, 3),
             "effective_GBps": round(bytes_job / el / 1e9, 1), "sweeps_per_step_imag": P_imag,
             "sweeps_per_step_real": P_real, "energy_final": e_final, "mu_final": mu_final,
             "variants": list(variants), "kernels": rows}

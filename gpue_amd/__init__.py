@@ -317,7 +317,7 @@ class Context:
 
     def sweep_profile(self, gstate, reps=50):
         """Per-kernel cost of the step: [{name, us, per_step}] from back-to-back
-        graph replays of each distinct sweep (in place on a scratch copy of psi), HIP events on the
+        graph replays of each distinct sweep (in place on fresh copies of psi, copies subtracted), HIP events on the
         context stream."""
         m = 32
         names = (ctypes.c_char_p * m)()

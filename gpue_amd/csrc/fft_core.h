@@ -250,31 +250,68 @@ struct TwE<float> {
 };
 
 template <typename T, int SIGN>
-__device__ __forceinline__ cx<T> twmul(cx<T> a, const typename TwE<T>::type* __restrict__ tw, int m) {
+__device__ __forceinline__ cx<T> twmul_val(cx<T> a, typename TwE<T>::type w) {
   if constexpr (sizeof(T) == 4) {
-    const float4 w = tw[m];
     const float sy = SIGN > 0 ? -1.0f : 1.0f;
     return cmul_split(a, w.x, sy * w.y, w.z, sy * w.w);
   } else {
-    cx<T> w = tw[m];
     if constexpr (SIGN > 0) w.y = -w.y;
     return cmul(a, w);
   }
 }
 
-// Compute pass p (radix R, L) on registers v[E] of thread tid.
-template <typename T, int N, int E, int R, int L, int SIGN>
-__device__ __forceinline__ void pass_compute(cx<T>* v, int tid, const typename TwE<T>::type* __restrict__ tw) {
+// Twiddles of one transform, per lane: pass P (L > 1) uses NB*(R-1) table
+// entries, stored from off(P).  They depend only on the lane, so a transform
+// issues all of its loads before its first pass: the table reads (L2 hits)
+// then overlap pass 0 and the LDS exchanges instead of stalling each pass.
+template <int N, int V, int ORD>
+struct TwPlan {
+  using S = Seq<N, V, ORD>;
+  static constexpr int NP = Plan<N, V>::NP, E = Plan<N, V>::E;
+  static constexpr int cnt(int P) { return S::Lbefore(P) > 1 ? (E / S::radix(P)) * (S::radix(P) - 1) : 0; }
+  static constexpr int off(int P) {
+    int o = 0;
+    for (int q = 0; q < P; ++q) o += cnt(q);
+    return o;
+  }
+  static constexpr int total() { return off(NP) > 0 ? off(NP) : 1; }
+};
+
+template <typename T, int N, int V, int ORD, int P = 0>
+__device__ __forceinline__ void tw_load(int tid, const typename TwE<T>::type* __restrict__ tw,
+                                        typename TwE<T>::type* w) {
+  using TP = TwPlan<N, V, ORD>;
+  if constexpr (P < TP::NP) {
+    constexpr int R = TP::S::radix(P), L = TP::S::Lbefore(P), E = TP::E, NB = E / R, TPL = N / E;
+    constexpr int STEP = N / (L * R);
+    if constexpr (L > 1) {
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        const int k = (tid + TPL * q) % L;
+#pragma unroll
+        for (int t = 1; t < R; ++t) w[TP::off(P) + q * (R - 1) + (t - 1)] = tw[t * k * STEP];
+      }
+    }
+    tw_load<T, N, V, ORD, P + 1>(tid, tw, w);
+  }
+}
+
+// Compute pass p (radix R, L) on registers v[E] of thread tid.  PRE: the
+// twiddles were loaded into w (from TwPlan::off(p)); otherwise they are read
+// from the table tw here (fewer registers live across the passes).
+template <typename T, int N, int E, int R, int L, int SIGN, bool PRE>
+__device__ __forceinline__ void pass_compute(cx<T>* v, int tid, const typename TwE<T>::type* __restrict__ tw,
+                                             const typename TwE<T>::type* w) {
   constexpr int TPL = N / E, NB = E / R;
   constexpr int STEP = N / (L * R);
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
     cx<T>* a = v + q * R;
     if constexpr (L > 1) {
-      const int j = tid + TPL * q;
-      const int k = j % L;
+      const int k = (tid + TPL * q) % L;
 #pragma unroll
-      for (int t = 1; t < R; ++t) a[t] = twmul<T, SIGN>(a[t], tw, t * k * STEP);
+      for (int t = 1; t < R; ++t)
+        a[t] = twmul_val<T, SIGN>(a[t], PRE ? w[q * (R - 1) + (t - 1)] : tw[t * k * STEP]);
     }
     DFT<T, R, SIGN>::run(a);
   }
@@ -303,10 +340,10 @@ struct LdsCol {
 // Full transform on registers: input in boundary layout of first radix,
 // output in boundary layout of last radix.  Inner passes exchange via LDS.
 // `active` predicates LDS traffic for padding threads (they still hit barriers).
-template <typename T, int N, int V, int ORD, int SIGN, int P, typename LA>
+template <typename T, int N, int V, int ORD, int SIGN, bool PRE, int P, typename LA>
 __device__ __forceinline__ void fft_from_pass(cx<T>* v, int tid, const typename TwE<T>::type* __restrict__ tw,
-                                              cx<T>* lds,
-                                              const LA& la, bool active) {
+                                              const typename TwE<T>::type* w, cx<T>* lds, const LA& la,
+                                              bool active) {
   using S = Seq<N, V, ORD>;
   constexpr int NP = Plan<N, V>::NP, E = Plan<N, V>::E, TPL = Plan<N, V>::TPL;
   if constexpr (P < NP) {
@@ -323,16 +360,20 @@ __device__ __forceinline__ void fft_from_pass(cx<T>* v, int tid, const typename 
         for (int r = 0; r < E; ++r) v[r] = lds[la.at(bpos<N, TPL, S::radix(P)>(tid, r))];
       }
     }
-    pass_compute<T, N, E, S::radix(P), S::Lbefore(P), SIGN>(v, tid, tw);
-    fft_from_pass<T, N, V, ORD, SIGN, P + 1>(v, tid, tw, lds, la, active);
+    pass_compute<T, N, E, S::radix(P), S::Lbefore(P), SIGN, PRE>(v, tid, tw, w + TwPlan<N, V, ORD>::off(P));
+    fft_from_pass<T, N, V, ORD, SIGN, PRE, P + 1>(v, tid, tw, w, lds, la, active);
   }
 }
 
-template <typename T, int N, int V, int ORD, int SIGN, typename LA>
+// PRE: load every twiddle of the transform before its first pass (for
+// latency-bound launches with registers to spare); otherwise per pass.
+template <typename T, int N, int V, int ORD, int SIGN, bool PRE, typename LA>
 __device__ __forceinline__ void fft_regs(cx<T>* v, int tid, const typename TwE<T>::type* __restrict__ tw, cx<T>* lds,
                                          const LA& la,
                                          bool active) {
-  fft_from_pass<T, N, V, ORD, SIGN, 0>(v, tid, tw, lds, la, active);
+  typename TwE<T>::type w[PRE ? TwPlan<N, V, ORD>::total() : 1];
+  if constexpr (PRE) tw_load<T, N, V, ORD>(tid, tw, w);
+  fft_from_pass<T, N, V, ORD, SIGN, PRE, 0>(v, tid, tw, w, lds, la, active);
 }
 
 }  // namespace gpue

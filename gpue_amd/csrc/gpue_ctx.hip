@@ -689,19 +689,20 @@ gpue_status gpue_energy(gpue_ctx* c, double* energy, double* mu) {
   return GPUE_OK;
 }
 
-// Time `reps` back-to-back launches of one sweep, captured in a graph and run in
-// place on a copy of psi in the scratch buffer (the state is untouched, the
-// footprint is the step's); returns microseconds per launch.
-static gpue_status time_sweep(gpue_ctx* c, Sweep sw, const Phys& ph, int reps, hipEvent_t e0, hipEvent_t e1,
+// Replays of a graph of `reps` (copy psi -> scratch, then optionally the sweep
+// in place on the scratch buffer) pairs; microseconds per replay.
+static gpue_status time_graph(gpue_ctx* c, const Sweep* sw, const Phys& ph, int reps, hipEvent_t e0, hipEvent_t e1,
                               double* us) {
-  sw.flags &= ~F_NORM;   // no reduction slots are written
-  HIPCHK(c, hipMemcpyAsync(c->tmp, c->psi, c->N * c->esz, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   gpue_status st = GPUE_OK;
-  for (int r = 0; r < reps && st == GPUE_OK; ++r) st = launch(c, sw, ph, c->tmp, c->tmp);
+  hipError_t e = hipSuccess;
+  for (int r = 0; r < reps && st == GPUE_OK && e == hipSuccess; ++r) {
+    e = launch_copy16(c->psi, c->tmp, c->N * c->esz, c->stream);
+    if (e == hipSuccess && sw) st = launch(c, *sw, ph, c->tmp, c->tmp);
+  }
   hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(c->stream, &g);
-  if (st != GPUE_OK || e != hipSuccess) {
+  hipError_t ec = hipStreamEndCapture(c->stream, &g);
+  if (st != GPUE_OK || e != hipSuccess || ec != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
     return st != GPUE_OK ? st : fail(c, GPUE_EHIP, "timing capture");
   }
@@ -717,7 +718,26 @@ static gpue_status time_sweep(gpue_ctx* c, Sweep sw, const Phys& ph, int reps, h
   if (ge) (void)hipGraphExecDestroy(ge);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("timing: ") + hipGetErrorString(e));
-  *us = ms * 1000.0 / (2.0 * reps);
+  *us = ms * 1000.0 / 2.0;
+  return GPUE_OK;
+}
+
+// Time one sweep as the step runs it: in place, on a state it has not touched
+// before.  Each of `reps` launches is preceded by a copy of psi into the
+// scratch buffer (psi is untouched), and the same graph without the sweeps is
+// timed and subtracted; returns microseconds per launch, including the
+// dispatch gap between dependent launches.  (Applied to its own output over
+// and over, an unnormalised transform would grow the values until real-time
+// phases overflow into the slow range of sincos; out of place, the 256^3
+// sweeps run 20-30 % slower than in place.)
+static gpue_status time_sweep(gpue_ctx* c, Sweep sw, const Phys& ph, int reps, hipEvent_t e0, hipEvent_t e1,
+                              double* us) {
+  sw.flags &= ~F_NORM;   // no reduction slots are written
+  double t_pair = 0.0, t_copy = 0.0;
+  gpue_status st = time_graph(c, &sw, ph, reps, e0, e1, &t_pair);
+  if (st == GPUE_OK) st = time_graph(c, nullptr, ph, reps, e0, e1, &t_copy);
+  if (st != GPUE_OK) return st;
+  *us = std::max(t_pair - t_copy, 0.0) / reps;
   return GPUE_OK;
 }
 
@@ -743,7 +763,7 @@ static gpue_status autotune(gpue_ctx* c, int reps) {
       Phys ph = make_phys(c, gstate, 0.0);
       for (Sweep s : sched) {
         for (int v = 0; v < N_VARIANTS && st == GPUE_OK; ++v) {
-          if (s.row && v > 1) break;   // tile width applies to strided sweeps only
+          if (s.row && v > 3) break;   // row sweeps: plan and twiddle loading only
           s.var = v;
           double us = 0.0;
           st = time_sweep(c, s, ph, reps, e0, e1, &us);

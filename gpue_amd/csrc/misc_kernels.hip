@@ -239,6 +239,21 @@ __global__ void k_vortex_resolve(const int* cand, int xDim, int* marker, int* fo
   if (local) atomicAdd(found, local);
 }
 
+// 16-byte copy (state snapshots for sweep timing); grid-stride
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ in, uint4* __restrict__ out, long n16) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) out[i] = in[i];
+}
+
+hipError_t launch_copy16(const void* in, void* out, size_t bytes, hipStream_t s) {
+  if (bytes % 16 != 0) return hipErrorInvalidValue;
+  const long n16 = (long)(bytes / 16);
+  const long want = (n16 + 255) / 256;
+  const int blocks = (int)(want < 8192 ? (want > 0 ? want : 1) : 8192);
+  hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, s, (const uint4*)in, (uint4*)out, n16);
+  return hipGetLastError();
+}
+
 __global__ void k_set_double(double* p, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
 }

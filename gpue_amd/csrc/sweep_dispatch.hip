@@ -24,7 +24,8 @@ bool sweep_supported(int N) {
 }
 
 // Kernel variant: bit 0 the plan (0 wide lines, 1 narrow lines); var >> 1
-// narrows the strided sweeps' tiles by 2^(var >> 1) (row sweeps ignore it).
+// narrows the strided sweeps' tiles by 2^(var >> 1); var >> 1 > 0 also loads
+// each transform's twiddles before its first pass (row and strided sweeps).
 hipError_t launch_sweep(int prec, bool row, int N, int var, int mode, const SweepArgs& a, hipStream_t s,
                         bool* handled) {
   *handled = true;

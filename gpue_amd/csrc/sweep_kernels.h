@@ -6,8 +6,11 @@ namespace gpue {
 
 // Workgroup shapes.  Row sweeps: LPW lines of TPL lanes.  Strided sweeps: TC
 // adjacent x-lines (lanes vary fastest in x for coalescing) of TPL lanes.
-template <int N, int V>
+// RV is the row-sweep variant: bit 0 the plan, bit 1 twiddles loaded before
+// the first pass of each transform (fft_regs PRE).
+template <int N, int RV>
 struct RowCfg {
+  static constexpr int V = RV & 1;
   static constexpr int TPL = Plan<N, V>::TPL;
   static constexpr int WGr = V == 0 ? 128 : 64;
   static constexpr int WG = TPL >= WGr ? TPL : WGr;
@@ -24,7 +27,9 @@ struct RowCfg {
 // to a half or a quarter of that width. Small grids have few tiles (64 of
 // them for 512 FP64 columns), so narrower tiles put more CUs to work; the
 // tiles that share a 128-byte line then run on one XCD (col_sweep), so the
-// line is fetched into one L2 only.
+// line is fetched into one L2 only.  Narrowed tiles also load each
+// transform's twiddles before its first pass (these launches are latency
+// bound and have registers to spare).
 template <typename T, int N, int CV>
 struct ColCfg {
   static constexpr int PV = CV & 1, NARROW = CV >> 1;
@@ -53,10 +58,12 @@ __device__ __forceinline__ double abs2d(cx<T> v) {
 // ---------------------------------------------------------------------------
 // Row sweep: lines are contiguous x-rows, line = y + yDim*z.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int V, int MODE>
+template <typename T, int N, int RV, int MODE>
 __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) {
+  constexpr int V = RV & 1;
+  constexpr bool PRE = (RV >> 1) != 0;
   using P = Plan<N, V>;
-  constexpr int TPL = P::TPL, E = P::E, LPW = RowCfg<N, V>::LPW;
+  constexpr int TPL = P::TPL, E = P::E, LPW = RowCfg<N, RV>::LPW;
   constexpr int FOOT = LdsRow::footprint(N);
   __shared__ cx<T> lds[P::NP > 1 ? LPW * FOOT : 1];
   __shared__ double sh[64];
@@ -82,12 +89,12 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 #pragma unroll
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(tid, r)];
     }
-    fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
+    fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
     if (flags & F_HALFV_POST) halfV_all<T, E>(p, v, pos0);
     if (flags & F_GAUGE_POST) {
-      fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
+      fft_regs<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, lds, la, active);
       gauge_all<T, E, 0>(p, v, posL, a.invN);
-      fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
+      fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
     }
   } else {
     if (active) {
@@ -102,18 +109,18 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
       }
       if (flags & F_HALFV_PRE) halfV_all<T, E>(p, v, pos0);
     }
-    fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
+    fft_regs<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, lds, la, active);
     if constexpr (MODE == RM_GAUGE || MODE == RM_GAUGE_FWD) {
       gauge_all<T, E, 0>(p, v, posL, a.invN);
-      fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
+      fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
     } else if constexpr (MODE == RM_FULL) {
       kstep_all<T, E>(p, v, posL, a.invN);
-      fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
+      fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
       if (flags & F_HALFV_POST) halfV_all<T, E>(p, v, pos0);
       if (flags & F_GAUGE_POST) {
-        fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
+        fft_regs<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, lds, la, active);
         gauge_all<T, E, 0>(p, v, posL, a.invN);
-        fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
+        fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
       }
     } else if constexpr (MODE == RM_KSUM) {
       if (active) {
@@ -129,7 +136,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
   // bridge modes continue with the next step's first row sweep
   if constexpr (MODE == RM_INV_FWD || MODE == RM_GAUGE_FWD) {
     if (flags & F_HALFV_PRE) halfV_all<T, E>(p, v, pos0);
-    fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
+    fft_regs<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, lds, la, active);
   }
 
   // store: forward sweeps leave the forward layout (last radix), the others the first radix
@@ -160,6 +167,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 template <typename T, int N, int CV, int MODE>
 __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) {
   constexpr int V = CV & 1;
+  constexpr bool PRE = (CV >> 1) != 0;
   using P = Plan<N, V>;
   constexpr int TC = ColCfg<T, N, CV>::TC, E = P::E, TPL = P::TPL;
   __shared__ cx<T> lds[P::NP > 1 ? TC * N : 1];
@@ -191,7 +199,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
 #pragma unroll
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(tid, r) * es];
     }
-    fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
+    fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
     if (active) {
 #pragma unroll
       for (int r = 0; r < E; ++r) out[bpos<N, TPL, RF0>(tid, r) * es] = v[r];
@@ -201,7 +209,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
 #pragma unroll
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RF0>(tid, r) * es];
     }
-    fft_regs<T, N, V, ORD_FWD, -1>(v, tid, tw, lds, la, active);
+    fft_regs<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, lds, la, active);
     if constexpr (MODE == CM_FWD) {
       if (active) {
 #pragma unroll
@@ -229,7 +237,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
         if (a.axis == 1) gauge_all<T, E, 1>(p, v, posL, a.invN);
         else gauge_all<T, E, 2>(p, v, posL, a.invN);
       }
-      fft_regs<T, N, V, ORD_REV, +1>(v, tid, tw, lds, la, active);
+      fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
       if (active) {
 #pragma unroll
         for (int r = 0; r < E; ++r) {

@@ -74,18 +74,21 @@ hipError_t launch_col_b_t(int mode, const SweepArgs* d, int ninst, int xDim, int
 }  // namespace gpue
 
 // One translation unit per (precision, N, plan) keeps the build parallel; it
-// holds the strided sweeps of every tile width (narrow = 0, 1, 2).
+// holds the strided sweeps of every tile width (narrow = 0, 1, 2) and the row
+// sweeps with per-pass or up-front twiddle loads (narrow = 0 or > 0).
 #define GPUE_SWEEP_TU(T_, N_, V_, NAME_)                                                          \
   namespace gpue {                                                                                \
   hipError_t NAME_(bool row, int narrow, int mode, const SweepArgs& a, hipStream_t s) {           \
-    if (row) return launch_row_t<T_, N_, V_>(mode, a, s);                                         \
+    if (row) return narrow ? launch_row_t<T_, N_, V_ + 2>(mode, a, s) : launch_row_t<T_, N_, V_>(mode, a, s); \
     if (narrow == 1) return launch_col_t<T_, N_, V_ + 2>(mode, a, s);                             \
     if (narrow == 2) return launch_col_t<T_, N_, V_ + 4>(mode, a, s);                             \
     return launch_col_t<T_, N_, V_>(mode, a, s);                                                  \
   }                                                                                               \
   hipError_t NAME_##_b(bool row, int narrow, int mode, const SweepArgs* d, int ninst, int lx,     \
                        int others, hipStream_t s) {                                               \
-    if (row) return launch_row_b_t<T_, N_, V_>(mode, d, ninst, lx, s);                            \
+    if (row)                                                                                      \
+      return narrow ? launch_row_b_t<T_, N_, V_ + 2>(mode, d, ninst, lx, s)                       \
+                    : launch_row_b_t<T_, N_, V_>(mode, d, ninst, lx, s);                          \
     if (narrow == 1) return launch_col_b_t<T_, N_, V_ + 2>(mode, d, ninst, lx, others, s);        \
     if (narrow == 2) return launch_col_b_t<T_, N_, V_ + 4>(mode, d, ninst, lx, others, s);        \
     return launch_col_b_t<T_, N_, V_>(mode, d, ninst, lx, others, s);                             \

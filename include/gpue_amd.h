@@ -130,12 +130,14 @@ gpue_status gpue_sync(gpue_ctx *c);
 
 /* FFT kernel shape per axis (no reference counterpart: cuFFT plans choose their
  * own).  gpue_autotune times every variant of every sweep of the current
- * configuration (graph replays of back-to-back launches on a scratch copy of
- * psi) and keeps the fastest per axis; variants[3] receives the choice
- * (0..5: bit 0 the plan, 0 = wide lines, 1 = narrow lines with fewer points
- * per lane; variant >> 1 = 0, 1, 2 narrows the tiles of the y/z sweeps to 1,
- * 1/2 or 1/4 of their default width, to spread small grids over more CUs;
- * the x axis uses bit 0 only).  gpue_set_variants forces a choice. */
+ * configuration (graph replays of back-to-back launches, each in place on a
+ * fresh copy of psi in the scratch buffer) and keeps the fastest per axis;
+ * variants[3] receives the choice (0..5: bit 0 the plan, 0 = wide lines,
+ * 1 = narrow lines with fewer points per lane; variant >> 1 = 0, 1, 2 narrows
+ * the tiles of the y/z sweeps to 1, 1/2 or 1/4 of their default width, to
+ * spread small grids over more CUs, and when > 0 each transform loads its
+ * twiddles before its first pass; the x axis takes 0..3, where 4, 5 act as
+ * 2, 3).  gpue_set_variants forces a choice. */
 gpue_status gpue_autotune(gpue_ctx *c, int reps, int *variants);
 gpue_status gpue_set_variants(gpue_ctx *c, const int *variants);
 /* Steady-state steps are captured as a HIP graph of `steps` consecutive steps
@@ -152,9 +154,10 @@ gpue_status gpue_timing_get(gpue_ctx *c, int kind, const char **name, long *laun
 /* Algorithmic HBM bytes of one step in the current configuration (psi sweeps). */
 gpue_status gpue_step_bytes(gpue_ctx *c, int gstate, double *bytes_per_step, double *sweeps_per_step);
 /* Per-kernel cost of the step (no reference counterpart): each distinct sweep
- * of the gstate schedule is launched `reps` times back to back from a graph, in
- * place on a copy of psi in the scratch buffer, and timed with HIP events on the
- * context stream.  Fills up to max_kinds entries: kernel name (owned by the
+ * of the gstate schedule is launched `reps` times back to back from a graph,
+ * each launch in place on a fresh copy of psi in the scratch buffer (psi is
+ * untouched; the copies are timed alone and subtracted), with HIP events on
+ * the context stream.  Fills up to max_kinds entries: kernel name (owned by the
  * ctx, valid until the next call), microseconds per launch, launches per step
  * (averaged over the two gauge orders); *n_kinds = number of distinct sweeps. */
 gpue_status gpue_sweep_profile(gpue_ctx *c, int gstate, int reps, int max_kinds, int *n_kinds,
