@@ -36,7 +36,7 @@ struct gpue_ctx {
   void* psi = nullptr;
   void* tmp = nullptr;
   double* dgrid[6] = {};
-  std::map<int, void*> tw;   // N -> twiddle table in the working precision
+  std::map<int, void*> tw;   // 2*N + plan -> pass twiddle table (TwTab) in the working precision
   int var[3] = {0, 0, 0};      // kernel variant per axis (row sweeps: axis 0)
   bool tuned = false;
   // steady-state step graphs: graph_steps consecutive steps (even count, starting

@@ -78,6 +78,9 @@ __device__ __forceinline__ cx<float> cmul_split(cx<float> a, float hx, float hy,
   return mk(fmaf(a.x, hx, re), fmaf(a.x, hy, im));
 }
 
+#ifndef GPUE_SPLIT_DFT
+#define GPUE_SPLIT_DFT 1
+#endif
 // multiply a by exp(SIGN*2*pi*i*m/R), with the exact special cases folded
 template <typename T, int R, int SIGN>
 __device__ __forceinline__ cx<T> twc(cx<T> a, int m) {
@@ -88,7 +91,7 @@ __device__ __forceinline__ cx<T> twc(cx<T> a, int m) {
   if (4 * m == 3 * R) return mul_si<T, -SIGN>(a);
   const int m48 = m * (48 / R);
   const double cd = Tw48::cosv(m48), sd = SIGN * Tw48::sinv(m48);
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 4 && GPUE_SPLIT_DFT) {
     const float ch = (float)cd, sh = (float)sd;
     return cmul_split(a, ch, sh, (float)(cd - (double)ch), (float)(sd - (double)sh));
   } else {
@@ -238,8 +241,9 @@ __device__ __forceinline__ constexpr int bpos(int tid, int r) {
   return tid + TPL * (r / R) + (r % R) * (N / R);
 }
 
-// Twiddle table: tw[m] = exp(-2*pi*i*m/N) (SIGN>0 uses the conjugate).  FP64:
-// one complex double per entry; FP32: (hi.re, hi.im, lo.re, lo.im) per entry.
+// Twiddle entries (tables laid out by TwTab): FP64 one complex double; FP32
+// (hi.re, hi.im, lo.re, lo.im), lo = the rounded remainder.  SIGN > 0
+// conjugates at use.
 template <typename T>
 struct TwE {
   using type = cx<T>;
@@ -260,10 +264,34 @@ __device__ __forceinline__ cx<T> twmul_val(cx<T> a, typename TwE<T>::type w) {
   }
 }
 
-// Twiddles of one transform, per lane: pass P (L > 1) uses NB*(R-1) table
-// entries, stored from off(P).  They depend only on the lane, so a transform
-// issues all of its loads before its first pass: the table reads (L2 hits)
-// then overlap pass 0 and the LDS exchanges instead of stalling each pass.
+// Pass twiddle table of a plan: for the forward order's passes, then the
+// reverse order's, every pass P with L > 1 holds L*(R-1) entries, entry
+// [t-1][k] = w^(t*k*STEP) with w = exp(-2*pi*i/N), STEP = N/(L*R) (the
+// inverse transform conjugates at use).  A lane's twiddles of a pass are
+// (t-1)*L entries apart from its own base: one address per pass, the loads
+// take immediate offsets, and for each t the lanes (consecutive k) read
+// consecutive entries.
+template <int N, int V>
+struct TwTab {
+  template <int ORD>
+  static constexpr int cnt(int P) {
+    return Seq<N, V, ORD>::Lbefore(P) > 1 ? Seq<N, V, ORD>::Lbefore(P) * (Seq<N, V, ORD>::radix(P) - 1) : 0;
+  }
+  template <int ORD>
+  static constexpr int off(int P) {
+    int o = 0;
+    if (ORD == ORD_REV)
+      for (int q = 0; q < Plan<N, V>::NP; ++q) o += cnt<ORD_FWD>(q);
+    for (int q = 0; q < P; ++q) o += cnt<ORD>(q);
+    return o;
+  }
+  static constexpr int total() { return off<ORD_REV>(Plan<N, V>::NP); }
+};
+
+// Twiddles a lane needs for a whole transform (PRE): pass P's NB*(R-1)
+// entries from off(P).  They depend only on the lane, so a transform can issue
+// all of its loads before its first pass: the table reads (L2 hits) then
+// overlap pass 0 and the LDS exchanges instead of stalling each pass.
 template <int N, int V, int ORD>
 struct TwPlan {
   using S = Seq<N, V, ORD>;
@@ -283,13 +311,13 @@ __device__ __forceinline__ void tw_load(int tid, const typename TwE<T>::type* __
   using TP = TwPlan<N, V, ORD>;
   if constexpr (P < TP::NP) {
     constexpr int R = TP::S::radix(P), L = TP::S::Lbefore(P), E = TP::E, NB = E / R, TPL = N / E;
-    constexpr int STEP = N / (L * R);
     if constexpr (L > 1) {
+      const typename TwE<T>::type* tp = tw + TwTab<N, V>::template off<ORD>(P);
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
         const int k = (tid + TPL * q) % L;
 #pragma unroll
-        for (int t = 1; t < R; ++t) w[TP::off(P) + q * (R - 1) + (t - 1)] = tw[t * k * STEP];
+        for (int t = 1; t < R; ++t) w[TP::off(P) + q * (R - 1) + (t - 1)] = tp[(t - 1) * L + k];
       }
     }
     tw_load<T, N, V, ORD, P + 1>(tid, tw, w);
@@ -298,20 +326,19 @@ __device__ __forceinline__ void tw_load(int tid, const typename TwE<T>::type* __
 
 // Compute pass p (radix R, L) on registers v[E] of thread tid.  PRE: the
 // twiddles were loaded into w (from TwPlan::off(p)); otherwise they are read
-// from the table tw here (fewer registers live across the passes).
+// here from the pass's table tp (fewer registers live across the passes).
 template <typename T, int N, int E, int R, int L, int SIGN, bool PRE>
-__device__ __forceinline__ void pass_compute(cx<T>* v, int tid, const typename TwE<T>::type* __restrict__ tw,
+__device__ __forceinline__ void pass_compute(cx<T>* v, int tid, const typename TwE<T>::type* __restrict__ tp,
                                              const typename TwE<T>::type* w) {
   constexpr int TPL = N / E, NB = E / R;
-  constexpr int STEP = N / (L * R);
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
     cx<T>* a = v + q * R;
     if constexpr (L > 1) {
       const int k = (tid + TPL * q) % L;
+      const typename TwE<T>::type* tk = tp + k;
 #pragma unroll
-      for (int t = 1; t < R; ++t)
-        a[t] = twmul_val<T, SIGN>(a[t], PRE ? w[q * (R - 1) + (t - 1)] : tw[t * k * STEP]);
+      for (int t = 1; t < R; ++t) a[t] = twmul_val<T, SIGN>(a[t], PRE ? w[q * (R - 1) + (t - 1)] : tk[(t - 1) * L]);
     }
     DFT<T, R, SIGN>::run(a);
   }
@@ -360,7 +387,8 @@ __device__ __forceinline__ void fft_from_pass(cx<T>* v, int tid, const typename 
         for (int r = 0; r < E; ++r) v[r] = lds[la.at(bpos<N, TPL, S::radix(P)>(tid, r))];
       }
     }
-    pass_compute<T, N, E, S::radix(P), S::Lbefore(P), SIGN, PRE>(v, tid, tw, w + TwPlan<N, V, ORD>::off(P));
+    pass_compute<T, N, E, S::radix(P), S::Lbefore(P), SIGN, PRE>(v, tid, tw + TwTab<N, V>::template off<ORD>(P),
+                                                                 w + TwPlan<N, V, ORD>::off(P));
     fft_from_pass<T, N, V, ORD, SIGN, PRE, P + 1>(v, tid, tw, w, lds, la, active);
   }
 }

@@ -100,10 +100,13 @@ static void fill_args(gpue_batch* b) {
               const Sweep& sw = on ? sc[k] : s.keys[k];
               SweepArgs& a = s.hargs[k][m];
               std::memset(&a, 0, sizeof(a));
-              a.active = on;
+              // every context holds the tables of both plans for each of its
+              // axes; a missing one would leave the sweep off rather than fault
+              const auto twi = c->tw.find(2 * c->dims[sw.axis] + (c->var[sw.axis] & 1));
+              a.active = on && twi != c->tw.end();
               a.in = c->psi;
               a.out = c->psi;
-              a.tw = c->tw[c->dims[sw.axis]];
+              a.tw = twi == c->tw.end() ? nullptr : twi->second;
               a.ph = phys;
               a.lines = c->dims[1] * c->dims[2];
               a.axis = sw.axis;

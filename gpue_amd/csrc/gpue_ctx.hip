@@ -29,6 +29,39 @@ using namespace gpue;
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
+template <int N, int V>
+static void radices_of(std::vector<int>& R) {
+  R.assign(Plan<N, V>::R, Plan<N, V>::R + Plan<N, V>::NP);
+}
+static bool plan_radices(int n, int plan, std::vector<int>& R) {
+  switch (n) {
+#define GPUE_RAD(N_) \
+  case N_: plan ? radices_of<N_, 1>(R) : radices_of<N_, 0>(R); return true;
+    GPUE_RAD(16) GPUE_RAD(32) GPUE_RAD(48) GPUE_RAD(64) GPUE_RAD(96) GPUE_RAD(128) GPUE_RAD(192) GPUE_RAD(256)
+    GPUE_RAD(384) GPUE_RAD(512) GPUE_RAD(1024)
+#undef GPUE_RAD
+    default: return false;
+  }
+}
+// Entry order of the device pass tables (TwTab in fft_core.h): the forward
+// radix order's passes, then the reverse order's; pass P with L > 1 gives
+// L*(R-1) entries [t-1][k] = index t*k*N/(L*R) of the full-circle table.
+static void pass_table(int n, const std::vector<int>& R, std::vector<int>& idx) {
+  idx.clear();
+  for (int ord = 0; ord < 2; ++ord) {
+    std::vector<int> seq = R;
+    if (ord) std::reverse(seq.begin(), seq.end());
+    int L = 1;
+    for (int r : seq) {
+      if (L > 1) {
+        const int step = n / (L * r);
+        for (int t = 1; t < r; ++t)
+          for (int k = 0; k < L; ++k) idx.push_back(t * k * step);
+      }
+      L *= r;
+    }
+  }
+}
 gpue_status gpue::fail(gpue_ctx* c, gpue_status s, const std::string& msg) {
   if (c) c->err = msg;
   return s;
@@ -185,7 +218,9 @@ gpue_status gpue::launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void
   a.active = 1;
   a.in = in;
   a.out = out;
-  a.tw = c->tw[n];
+  const auto twi = c->tw.find(2 * n + (var & 1));
+  if (twi == c->tw.end()) return fail(c, GPUE_EUNSUPPORTED, "no twiddle table for n=" + std::to_string(n));
+  a.tw = twi->second;
   a.ph = ph;
   a.lines = c->dims[1] * c->dims[2];
   a.axis = s.axis;
@@ -421,30 +456,37 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
     if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
   }
   for (int a = 0; a < p.dimnum; ++a) {
-    int n = c->dims[a];
-    if (c->tw.count(n)) continue;
+    const int n = c->dims[a];
     std::vector<double> re, im;
     twiddles(n, re, im);
-    void* t = nullptr;
-    if (hipMalloc(&t, (size_t)n * 16) != hipSuccess) return bail(GPUE_ENOMEM, "twiddles");
-    c->tw[n] = t;
-    if (c->prec == 64) {
-      std::vector<double> il(2 * n);
-      for (int m = 0; m < n; ++m) { il[2 * m] = re[m]; il[2 * m + 1] = im[m]; }
-      e = hipMemcpy(t, il.data(), il.size() * sizeof(double), hipMemcpyHostToDevice);
-    } else {
-      // split twiddles: (hi.re, hi.im, lo.re, lo.im), lo = the rounded remainder
-      std::vector<float> il(4 * n);
-      for (int m = 0; m < n; ++m) {
-        const float hr = (float)re[m], hi = (float)im[m];
-        il[4 * m] = hr;
-        il[4 * m + 1] = hi;
-        il[4 * m + 2] = (float)(re[m] - (double)hr);
-        il[4 * m + 3] = (float)(im[m] - (double)hi);
+    for (int plan = 0; plan < 2; ++plan) {
+      if (c->tw.count(2 * n + plan)) continue;
+      std::vector<int> R;
+      if (!plan_radices(n, plan, R)) return bail(GPUE_EUNSUPPORTED, "no plan for n");
+      std::vector<int> idx;
+      pass_table(n, R, idx);
+      void* t = nullptr;
+      if (hipMalloc(&t, std::max<size_t>(idx.size(), 1) * 16) != hipSuccess) return bail(GPUE_ENOMEM, "twiddles");
+      c->tw[2 * n + plan] = t;
+      if (c->prec == 64) {
+        std::vector<double> il(2 * idx.size());
+        for (size_t j = 0; j < idx.size(); ++j) { il[2 * j] = re[idx[j]]; il[2 * j + 1] = im[idx[j]]; }
+        e = hipMemcpy(t, il.data(), il.size() * sizeof(double), hipMemcpyHostToDevice);
+      } else {
+        // split twiddles: (hi.re, hi.im, lo.re, lo.im), lo = the rounded remainder
+        std::vector<float> il(4 * idx.size());
+        for (size_t j = 0; j < idx.size(); ++j) {
+          const double xr = re[idx[j]], xi = im[idx[j]];
+          const float hr = (float)xr, hi = (float)xi;
+          il[4 * j] = hr;
+          il[4 * j + 1] = hi;
+          il[4 * j + 2] = (float)(xr - (double)hr);
+          il[4 * j + 3] = (float)(xi - (double)hi);
+        }
+        e = hipMemcpy(t, il.data(), il.size() * sizeof(float), hipMemcpyHostToDevice);
       }
-      e = hipMemcpy(t, il.data(), il.size() * sizeof(float), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return bail(GPUE_EHIP, "twiddle upload");
     }
-    if (e != hipSuccess) return bail(GPUE_EHIP, "twiddle upload");
   }
   // reduction scratch: the largest grid of any reducing kernel, times 4 accumulators
   size_t nb = REDUCE_BLOCKS;
