@@ -308,8 +308,9 @@ def main():
                     help="profiling: run one workload only (copy: a plain 256 MiB device copy, the "
                          "calibration kernel for the HBM byte counters)")
     ap.add_argument("--steps3", type=int, default=200, help="timed 256^3 steps per precision")
-    ap.add_argument("--graph-steps", type=int, default=8,
-                    help="steps per captured graph (0: every sweep launched individually, e.g. under a tracer)")
+    ap.add_argument("--graph-steps", type=int, default=32,
+                    help="steps per captured graph (library default 8; 32 saves ~1.5 %% of graph launch "
+                         "gaps here; 0: every sweep launched individually, e.g. under a tracer)")
     ap.add_argument("--variants", type=str, default=None,
                     help="kernel variants per axis, e.g. 1,1,1 (default: autotune)")
     args = ap.parse_args()
