@@ -64,10 +64,11 @@ def make_ctx(G, prec, cfg, seed=1, phase=False, noise=0.01):
     return ctx, psi
 
 
-def tune(ctx, args):
+def tune(ctx, args, prec=64):
     ctx.set_graph_steps(args.graph_steps)
-    if args.variants:
-        v = [int(a) for a in args.variants.split(",")]
+    spec = args.variants32 if (prec == 32 and args.variants32) else args.variants
+    if spec:
+        v = [int(a) for a in spec.split(",")]
         ctx.set_variants(v)
         return tuple(v)
     return ctx.autotune()
@@ -84,31 +85,21 @@ def run_schedule(ctx, imag, real):
 # (each scaled by its reading for a plain 256 MiB device copy: 2.0 and 1.0), per
 # kernel instance and workload; the passes and tables are in profiles/.
 MEASURED_TRAFFIC = {
-    "gpue::row_sweep<double, 512, 1, 0>@512sq": 8704940,
-    "gpue::row_sweep<double, 512, 1, 1>@512sq": 8842309,
-    "gpue::row_sweep<double, 512, 1, 2>@512sq": 8730191,
-    "gpue::col_sweep<double, 512, 1, 2>@512sq": 8737358,
-    "gpue::col_sweep<double, 512, 1, 3>@512sq": 8735311,
-    "gpue::row_sweep<double, 256, 0, 0>@256cube": 537486839,
-    "gpue::row_sweep<double, 256, 1, 0>@256cube": 537160228,
-    "gpue::row_sweep<double, 256, 0, 1>@256cube": 537553222,
-    "gpue::row_sweep<double, 256, 1, 1>@256cube": 537382268,
-    "gpue::col_sweep<double, 256, 0, 0>@256cube": 537019155,
-    "gpue::col_sweep<double, 256, 1, 0>@256cube": 536958248,
-    "gpue::col_sweep<double, 256, 0, 1>@256cube": 537015959,
-    "gpue::col_sweep<double, 256, 1, 1>@256cube": 536966591,
-    "gpue::col_sweep<double, 256, 0, 2>@256cube": 537289281,
-    "gpue::col_sweep<double, 256, 1, 2>@256cube": 537191259,
-    "gpue::row_sweep<float, 256, 0, 0>@256cube": 268777199,
-    "gpue::row_sweep<float, 256, 1, 0>@256cube": 268715247,
-    "gpue::row_sweep<float, 256, 0, 1>@256cube": 268841374,
-    "gpue::row_sweep<float, 256, 1, 1>@256cube": 268747861,
-    "gpue::col_sweep<float, 256, 0, 0>@256cube": 268573582,
-    "gpue::col_sweep<float, 256, 1, 0>@256cube": 268520444,
-    "gpue::col_sweep<float, 256, 0, 1>@256cube": 268580273,
-    "gpue::col_sweep<float, 256, 1, 1>@256cube": 268528106,
-    "gpue::col_sweep<float, 256, 0, 2>@256cube": 268802270,
-    "gpue::col_sweep<float, 256, 1, 2>@256cube": 268741749,
+    "gpue::col_sweep<double, 512, 5, 2>@512sq": 8803333,
+    "gpue::col_sweep<double, 512, 5, 3>@512sq": 8799003,
+    "gpue::row_sweep<double, 512, 3, 0>@512sq": 8676618,
+    "gpue::row_sweep<double, 512, 3, 1>@512sq": 8858157,
+    "gpue::row_sweep<double, 512, 3, 2>@512sq": 8810723,
+    "gpue::col_sweep<double, 256, 1, 0>@256cube": 536974439,
+    "gpue::col_sweep<double, 256, 1, 1>@256cube": 536981941,
+    "gpue::col_sweep<double, 256, 2, 2>@256cube": 537384725,
+    "gpue::col_sweep<float, 256, 0, 0>@256cube": 268574894,
+    "gpue::col_sweep<float, 256, 0, 1>@256cube": 268569892,
+    "gpue::col_sweep<float, 256, 0, 2>@256cube": 268867855,
+    "gpue::row_sweep<double, 256, 3, 0>@256cube": 537111506,
+    "gpue::row_sweep<double, 256, 3, 1>@256cube": 537705260,
+    "gpue::row_sweep<float, 256, 0, 0>@256cube": 268643041,
+    "gpue::row_sweep<float, 256, 0, 1>@256cube": 268839678,
 }
 
 RM = {"row_fwd": 0, "row_inv": 1, "row_gauge": 2, "row_full": 3, "row_ksum": 4, "row_inv_fwd": 5,
@@ -311,6 +302,8 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=32,
                     help="steps per captured graph (library default 8; 32 saves ~1.5 %% of graph launch "
                          "gaps here; 0: every sweep launched individually, e.g. under a tracer)")
+    ap.add_argument("--variants32", type=str, default=None,
+                    help="kernel variants of the FP32 contexts (default: as --variants)")
     ap.add_argument("--variants", type=str, default=None,
                     help="kernel variants per axis, e.g. 1,1,1 (default: autotune)")
     args = ap.parse_args()
@@ -382,7 +375,7 @@ def main():
         res3 = {}
         for prec in (64, 32):
             ctx, psi0 = make_ctx(G, prec, CFG3, seed=1)
-            variants = tune(ctx, args)
+            variants = tune(ctx, args, prec)
             ctx.evolve(10, 0)
             ctx.upload(psi0)
             ctx.sync()

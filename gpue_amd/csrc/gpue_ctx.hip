@@ -764,17 +764,48 @@ static gpue_status time_graph(gpue_ctx* c, const Sweep* sw, const Phys& ph, int 
   return GPUE_OK;
 }
 
-// Time one sweep as the step runs it: in place, on a state it has not touched
-// before.  Each of `reps` launches is preceded by a copy of psi into the
-// scratch buffer (psi is untouched), and the same graph without the sweeps is
-// timed and subtracted; returns microseconds per launch, including the
-// dispatch gap between dependent launches.  (Applied to its own output over
-// and over, an unnormalised transform would grow the values until real-time
-// phases overflow into the slow range of sincos; out of place, the 256^3
-// sweeps run 20-30 % slower than in place.)
+// Time one sweep as the step runs it, in place; returns microseconds per
+// launch, including the dispatch gap between dependent launches.
+//  * States that fit the L2s (<= 32 MiB), and real time: each of `reps`
+//    launches is preceded by a copy of psi into the scratch buffer, and the
+//    same graph without the sweeps is timed and subtracted.  A launch on its
+//    own output would find its lines in the L2 of the XCD that wrote them,
+//    which no step does, and real-time phases of an unnormalised transform
+//    applied over and over grow into the slow range of sincos.
+//  * Larger states in imaginary time (no data-dependent path): `reps`
+//    launches in place on one copy.  Here the subtraction over-counts (a
+//    copy after a sweep runs slower than after a copy): measured against
+//    rocprofv3 kernel durations at 256^3, in place is within 2-5 %, the
+//    subtraction 7-25 % above.
 static gpue_status time_sweep(gpue_ctx* c, Sweep sw, const Phys& ph, int reps, hipEvent_t e0, hipEvent_t e1,
                               double* us) {
   sw.flags &= ~F_NORM;   // no reduction slots are written
+  if (c->N * c->esz > ((size_t)32 << 20) && ph.gstate == 0) {
+    HIPCHK(c, hipMemcpyAsync(c->tmp, c->psi, c->N * c->esz, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    gpue_status st = GPUE_OK;
+    for (int r = 0; r < reps && st == GPUE_OK; ++r) st = launch(c, sw, ph, c->tmp, c->tmp);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (st != GPUE_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return st != GPUE_OK ? st : fail(c, GPUE_EHIP, "timing capture");
+    }
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(ge, c->stream);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    for (int r = 0; r < 2 && e == hipSuccess; ++r) e = hipGraphLaunch(ge, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("timing: ") + hipGetErrorString(e));
+    *us = ms * 1000.0 / (2.0 * reps);
+    return GPUE_OK;
+  }
   double t_pair = 0.0, t_copy = 0.0;
   gpue_status st = time_graph(c, &sw, ph, reps, e0, e1, &t_pair);
   if (st == GPUE_OK) st = time_graph(c, nullptr, ph, reps, e0, e1, &t_copy);
@@ -1165,10 +1196,8 @@ gpue_status gpue_timing_get(gpue_ctx* c, int kind, const char** name, long* laun
 }
 
 // Per-kernel cost: every distinct sweep of the gstate schedules (both parities)
-// is captured `reps` times back to back in a graph and timed with HIP events on
-// the context stream over three replays.  The launches run in place on a copy
-// of psi in the scratch buffer (the same footprint as the step, the state is
-// untouched); the norm partials are not written.
+// is timed by time_sweep (graph replays in place on copies of psi in the
+// scratch buffer; the state is untouched, the norm partials are not written).
 gpue_status gpue_sweep_profile(gpue_ctx* c, int gstate, int reps, int max_kinds, int* n_kinds, const char** names,
                                double* us_per_launch, double* launches_per_step) {
   if (!c || (gstate != 0 && gstate != 1) || reps <= 0 || max_kinds < 0 || !n_kinds) return GPUE_EINVAL;

@@ -130,8 +130,9 @@ gpue_status gpue_sync(gpue_ctx *c);
 
 /* FFT kernel shape per axis (no reference counterpart: cuFFT plans choose their
  * own).  gpue_autotune times every variant of every sweep of the current
- * configuration (graph replays of back-to-back launches, each in place on a
- * fresh copy of psi in the scratch buffer) and keeps the fastest per axis;
+ * configuration (graph replays of back-to-back launches in place on copies
+ * of psi, as gpue_sweep_profile) and keeps the fastest per axis, then refines
+ * the choice over whole graph-replayed steps on a copy of the state;
  * variants[3] receives the choice (0..5: bit 0 the plan, 0 = wide lines,
  * 1 = narrow lines with fewer points per lane; variant >> 1 = 0, 1, 2 narrows
  * the tiles of the y/z sweeps to 1, 1/2 or 1/4 of their default width, to
@@ -155,9 +156,10 @@ gpue_status gpue_timing_get(gpue_ctx *c, int kind, const char **name, long *laun
 gpue_status gpue_step_bytes(gpue_ctx *c, int gstate, double *bytes_per_step, double *sweeps_per_step);
 /* Per-kernel cost of the step (no reference counterpart): each distinct sweep
  * of the gstate schedule is launched `reps` times back to back from a graph,
- * each launch in place on a fresh copy of psi in the scratch buffer (psi is
- * untouched; the copies are timed alone and subtracted), with HIP events on
- * the context stream.  Fills up to max_kinds entries: kernel name (owned by the
+ * in place in the scratch buffer (psi is untouched): each launch on a fresh
+ * copy of psi, the copies timed alone and subtracted, except for states over
+ * 32 MiB in imaginary time, which run on one copy; HIP events on the context
+ * stream.  Fills up to max_kinds entries: kernel name (owned by the
  * ctx, valid until the next call), microseconds per launch, launches per step
  * (averaged over the two gauge orders); *n_kinds = number of distinct sweeps. */
 gpue_status gpue_sweep_profile(gpue_ctx *c, int gstate, int reps, int max_kinds, int *n_kinds,
