@@ -52,6 +52,8 @@ struct gpue_ctx {
   double* vs[3] = {};          // separable V / K value tables
   double* ks[3] = {};
   double* kf[2][3] = {};       // per-axis K operator factors for gstate 0 / 1
+  float* vsf[3] = {};          // FP32 contexts: vs and kf rounded to FP32
+  float* kff[2][3] = {};
   double* Vt = nullptr;
   double* Kt = nullptr;
   double* At[3] = {};

@@ -92,6 +92,14 @@ __device__ __forceinline__ cx<T> twc(cx<T> a, int m) {
   const int m48 = m * (48 / R);
   const double cd = Tw48::cosv(m48), sd = SIGN * Tw48::sinv(m48);
   if constexpr (sizeof(T) == 4 && GPUE_SPLIT_DFT) {
+    if ((8 * m) % R == 0) {
+      // odd multiple of 45 degrees: w = c*(sx + i*sy) with c = sqrt(1/2) split,
+      // sx, sy = +-1: one exact sign pattern, then two split scalings
+      const float sx = cd > 0 ? 1.0f : -1.0f, sy = sd > 0 ? 1.0f : -1.0f;
+      const float ch = 0.70710677f, cl = (float)(0.70710678118654752440 - (double)0.70710677f);
+      const float u = sx * a.x - sy * a.y, w = sx * a.y + sy * a.x;
+      return mk(fmaf(u, ch, u * cl), fmaf(w, ch, w * cl));
+    }
     const float ch = (float)cd, sh = (float)sd;
     return cmul_split(a, ch, sh, (float)(cd - (double)ch), (float)(sd - (double)sh));
   } else {

@@ -82,6 +82,15 @@ static double* dev_copy(const std::vector<double>& v, hipError_t* e) {
   return p;
 }
 
+static float* dev_copy_f(const std::vector<double>& v, hipError_t* e) {
+  std::vector<float> h(v.begin(), v.end());   // rounded once, to nearest
+  float* p = nullptr;
+  *e = hipMalloc((void**)&p, std::max<size_t>(1, h.size()) * sizeof(float));
+  if (*e != hipSuccess) return nullptr;
+  *e = hipMemcpy(p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+  return p;
+}
+
 Phys gpue::make_phys(const gpue_ctx* c, int gstate, double time) {
   const gpue_params& p = c->p;
   Phys f;
@@ -116,6 +125,8 @@ Phys gpue::make_phys(const gpue_ctx* c, int gstate, double time) {
     (&f.xg)[a] = c->dgrid[a];
     (&f.pxg)[a] = c->dgrid[a + 3];
     f.vs[a] = c->vs[a];
+    f.vsf[a] = c->vsf[a];
+    f.kff[a] = c->kff[gstate][a];
     f.ks[a] = c->ks[a];
     f.kf[a] = c->kf[gstate][a];
   }
@@ -373,7 +384,8 @@ void gpue_destroy(gpue_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* bufs[] = {c->psi, c->tmp, c->partials, c->results, c->Vt, c->Kt, c->At[0], c->At[1], c->At[2],
                   c->vs[0], c->vs[1], c->vs[2], c->ks[0], c->ks[1], c->ks[2], c->kf[0][0], c->kf[0][1],
-                  c->kf[0][2], c->kf[1][0], c->kf[1][1], c->kf[1][2],
+                  c->kf[0][2], c->kf[1][0], c->kf[1][1], c->kf[1][2], c->vsf[0], c->vsf[1], c->vsf[2],
+                  c->kff[0][0], c->kff[0][1], c->kff[0][2], c->kff[1][0], c->kff[1][1], c->kff[1][2],
                   c->vmask, c->vcand, c->vmarker, c->vfound};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -454,6 +466,14 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
     if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
     c->kf[1][a] = dev_copy(fe, &e);
     if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
+    if (c->prec == 32) {
+      c->vsf[a] = dev_copy_f(vsh[a], &e);
+      if (e != hipSuccess) return bail(GPUE_ENOMEM, "V tables");
+      c->kff[0][a] = dev_copy_f(fg, &e);
+      if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
+      c->kff[1][a] = dev_copy_f(fe, &e);
+      if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
+    }
   }
   for (int a = 0; a < p.dimnum; ++a) {
     const int n = c->dims[a];

@@ -65,6 +65,8 @@ struct Phys {
   const double *ks[3];   // K_SEP: K = ks[0][kx] + ks[1][ky] + ks[2][kz]
   const double *kf[3];   // K_SEP: per-axis factors of the K operator for this gstate,
                          // interleaved (re, im): exp(-K_a Dt/hbar) or exp(-i K_a Dt/hbar)
+  const float *vsf[3];   // FP32 contexts: vs and kf rounded to FP32 once (the FP32
+  const float *kff[3];   // sweeps read these; null for FP64 contexts)
   const double *Vt, *Kt, *At[3];
 };
 
@@ -260,7 +262,10 @@ struct WMath<double> {
 };
 template <>
 struct WMath<float> {
-  static __device__ __forceinline__ float ex(float a) { return expf(a); }
+  // hardware exp2 of a*log2(e) (__expf): the FP32 step rounds its operands to
+  // FP32 anyway, and |a| stays small (V dt/2, K dt), so the product's rounding
+  // adds |a|*2^-24 relative
+  static __device__ __forceinline__ float ex(float a) { return __expf(a); }
   static __device__ __forceinline__ void sc(float a, float* s, float* c) { sincos_cw(a, s, c); }
   static __device__ __forceinline__ void sc_core(float a, float* s, float* c) { sincos_core(a, s, c); }
   static __device__ __forceinline__ bool sc_ok(float a) { return sincos_fast_ok(a); }
@@ -327,7 +332,10 @@ __device__ __forceinline__ void halfV_all(const Phys& p, cx<T>* v, F pos) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const Pt q = pos(r);
-      arg[r] = (T)(p.vs[0][q.x] + p.vs[1][q.y] + p.vs[2][q.z]);
+      if constexpr (sizeof(T) == 4)
+        arg[r] = p.vsf[0][q.x] + p.vsf[1][q.y] + p.vsf[2][q.z];
+      else
+        arg[r] = p.vs[0][q.x] + p.vs[1][q.y] + p.vs[2][q.z];
     }
   } else {
 #pragma unroll
@@ -383,6 +391,16 @@ __device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double
   apply_phase_all<T, E>(v, arg, p.gstate, invn);
 }
 
+// Per-axis K factor table in the working precision.
+template <typename T>
+struct KTab {
+  static __device__ __forceinline__ const double* get(const Phys& p, int a) { return p.kf[a]; }
+};
+template <>
+struct KTab<float> {
+  static __device__ __forceinline__ const float* get(const Phys& p, int a) { return p.kff[a]; }
+};
+
 // K step: GK/EK (src/operators.cu:1082,1102) with both 1/sqrt(N) of
 // src/evolution.cu:761,777 folded into invN.  For the separable K the operator
 // is the product of per-axis factors (no transcendental per point).
@@ -394,9 +412,9 @@ __device__ __forceinline__ void kstep_all(const Phys& p, cx<T>* v, F pos, double
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const Pt q = pos(r);
-      const double* fx = p.kf[0] + 2 * q.x;
-      const double* fy = p.kf[1] + 2 * q.y;
-      const double* fz = p.kf[2] + 2 * q.z;
+      const auto* fx = KTab<T>::get(p, 0) + 2 * q.x;
+      const auto* fy = KTab<T>::get(p, 1) + 2 * q.y;
+      const auto* fz = KTab<T>::get(p, 2) + 2 * q.z;
       f[r][0] = (T)fx[0]; f[r][1] = (T)fx[1]; f[r][2] = (T)fy[0];
       f[r][3] = (T)fy[1]; f[r][4] = (T)fz[0]; f[r][5] = (T)fz[1];
     }
@@ -434,10 +452,11 @@ __device__ __forceinline__ void kstep_col(const Phys& p, cx<T>* v, int xi, int o
     return;
   }
   const T invN = (T)invN_;
-  const T fx0 = (T)p.kf[0][2 * xi], fx1 = (T)p.kf[0][2 * xi + 1];
-  const double* fop = p.kf[AXIS == 1 ? 2 : 1] + 2 * o;
+  const auto* fxp = KTab<T>::get(p, 0) + 2 * xi;
+  const T fx0 = (T)fxp[0], fx1 = (T)fxp[1];
+  const auto* fop = KTab<T>::get(p, AXIS == 1 ? 2 : 1) + 2 * o;
   const T fo0 = (T)fop[0], fo1 = (T)fop[1];
-  const double* fvp = p.kf[AXIS];
+  const auto* fvp = KTab<T>::get(p, AXIS);
   T fe[E][2];
 #pragma unroll
   for (int r = 0; r < E; ++r) {
