@@ -515,8 +515,13 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
     if (p.dimnum > 1) nb = std::max<size_t>(nb, sweep_blocks(c->prec, false, p.yDim, v, p.xDim, p.zDim));
     if (p.dimnum > 2) nb = std::max<size_t>(nb, sweep_blocks(c->prec, false, p.zDim, v, p.xDim, p.yDim));
   }
-  // default variant: narrow lines for small (latency-bound) grids, wide ones otherwise
-  for (int a = 0; a < 3; ++a) c->var[a] = c->N <= (1L << 20) ? 1 : 0;
+  // default variants (no timing, so results never depend on it): for small,
+  // latency-bound grids narrow lines with up-front twiddle loads, and quarter
+  // strided tiles (the autotuner's choice for 2D grids of 128^2-512^2); wide
+  // lines otherwise
+  const bool small = c->N <= (1L << 20);
+  c->var[0] = small ? 3 : 0;
+  c->var[1] = c->var[2] = small ? 5 : 0;
   c->partials_n = nb * 4;
   if (hipMalloc((void**)&c->partials, c->partials_n * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->results, 16 * sizeof(double)) != hipSuccess)
