@@ -294,3 +294,51 @@ def test_ramp_matches_oracle(name, dim, kw, ramp_type):
             ref.step(i, gstate)
         assert rel_l2(ref.wfc(), orc.wfc()) > 1e-6
         ctx.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("name,dim,kw", [c for c in CASES if c[0] in ("2d-gauge", "3d-gauge")],
+                         ids=["2d-gauge", "3d-gauge"])
+def test_every_kernel_variant_fp32(name, dim, kw, variant):
+    # the FP32 sweeps of every variant (split twiddles, 45-degree rotations,
+    # FP32 tables) meet the FP32 bar in imaginary and real time
+    for gstate in (0, 1):
+        ctx = G.Context(precision=32, gdt=1e-3, dt=5e-4, **kw)
+        ctx.set_variants([variant] * 3)
+        orc = O.Oracle(gdt=1e-3, dt=5e-4, **{k: v for k, v in kw.items() if k != "a_kind"})
+        psi = seeded_psi(ctx.shape, [ctx.grid(a) for a in range(ctx.dimnum)], seed=4, phase=True)
+        psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
+        ctx.upload(psi)
+        orc.set_wfc(psi)
+        ctx.evolve(20, gstate)
+        for i in range(20):
+            orc.step(i, gstate)
+        assert rel_l2(ctx.download(), orc.wfc()) <= TOL32_PSI, (variant, gstate)
+        (eg, mg), (eo, mo) = ctx.energy(), orc.energy()
+        assert abs(eg - eo) <= TOL32_E * abs(eo) and abs(mg - mo) <= TOL32_E * abs(mo), (variant, gstate)
+        ctx.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_autotune_leaves_state(prec):
+    # the autotuner times sweeps and whole steps on copies: after it, the run
+    # continues exactly as one that was given the chosen variants directly
+    kw = dict(dimnum=2, xDim=128, yDim=128, xMax=9, yMax=9, g=80, gpe=1, corotating=1, omega=0.4,
+              gdt=1e-3, dt=5e-4)
+    a, b = G.Context(precision=prec, **kw), G.Context(precision=prec, **kw)
+    psi = seeded_psi(a.shape, [a.grid(k) for k in range(2)], seed=6, phase=True)
+    for c in (a, b):
+        c.upload(psi)
+        c.evolve(10, 0)   # leaves a norm pending
+    v = a.autotune()
+    b.set_variants(list(v))
+    a.set_variants(list(v))
+    wa, wb = a.download(), b.download()   # both settle the pending norm the same way
+    assert np.array_equal(wa, wb)
+    for c in (a, b):
+        c.evolve(16, 0, first_step=10)
+        c.evolve(16, 1, first_step=26)
+    assert np.array_equal(a.download(), b.download())
+    assert a.energy() == b.energy()
+    a.close()
+    b.close()
