@@ -896,6 +896,7 @@ static gpue_status refine_steps(gpue_ctx* c, hipEvent_t e0, hipEvent_t e1) {
     }
   }
   std::swap(c->psi, c->tmp);
+  drop_graphs(c);   // captured on the copy (also after a failed trial)
   for (int a = 0; a < 3; ++a) c->var[a] = best[a];
   c->next_step = next_step;
   c->pending_norm = false;   // as at entry (the tuner settles the pending norm first)
