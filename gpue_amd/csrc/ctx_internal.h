@@ -18,6 +18,7 @@ struct Sweep {
   unsigned flags;
   double invN;
   int var = -1;   // kernel variant; -1: the context's choice for this axis
+  int part_nb = 0;   // F_SCALE_PART: partials left by the previous sweep
 };
 
 constexpr int TIMING_PAIRS = 256;
@@ -64,6 +65,7 @@ struct gpue_ctx {
   gpue::EqnProg veq;
   bool pending_norm = false;   // results[0] holds a norm not yet applied (else it holds 1.0)
   int next_step = 0;           // step index after the last evolve (batch parity)
+  int defer_nb = 0;            // eager steps: a finalize left to the next step's first sweep
   double last_energy = 0.0;
   std::string err;
   // vortex scratch

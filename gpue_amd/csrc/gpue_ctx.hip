@@ -221,6 +221,41 @@ static gpue_status timing_flush(gpue_ctx* c) {
   return GPUE_OK;
 }
 
+// Workgroups (= partials written) of a sweep.
+static int sweep_nb(const gpue_ctx* c, const Sweep& s) {
+  const int n = c->dims[s.axis];
+  const int var = s.var >= 0 ? s.var : c->var[s.axis];
+  return s.row ? sweep_blocks(c->prec, true, n, var, c->dims[1] * c->dims[2], 1)
+               : sweep_blocks(c->prec, false, n, var, c->dims[0], s.axis == 1 ? c->dims[2] : c->dims[1]);
+}
+
+// Deferred renormalisation: the norm sweep `last` of one imaginary step leaves
+// its partials to `next`, the first sweep of the following step, which reduces
+// them itself (part_norm, the same additions as the finalize) instead of a
+// one-workgroup finalize kernel and its dispatch gap between the two.  Only
+// for fewer than 4096 partials (every workgroup of `next` sums all of them)
+// and when `next` is the row sweep that scales its input.
+static bool defer_norm(const gpue_ctx* c, Sweep& last, Sweep& next) {
+  if (c->p.dimnum < 2 || !(last.flags & F_NORM) || !(next.flags & F_SCALE_IN) || !next.row || next.mode != RM_FWD)
+    return false;
+  const int nb = sweep_nb(c, last);
+  if (nb >= 4096) return false;
+  last.flags |= F_DEFER;
+  next.flags |= F_SCALE_PART;
+  next.part_nb = nb;
+  return true;
+}
+
+// A finalize left by an eager step whose successor did not take it.
+static gpue_status settle_deferred(gpue_ctx* c) {
+  if (c->defer_nb <= 0) return GPUE_OK;
+  const double dV = c->d[0] * c->d[1] * c->d[2];
+  hipError_t e = launch_finalize(c->partials, c->defer_nb, 1, 1, dV, c->results + 0, c->stream);
+  c->defer_nb = 0;
+  if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("finalize: ") + hipGetErrorString(e));
+  return GPUE_OK;
+}
+
 gpue_status gpue::launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void* in, void* out) {
   const int n = c->dims[s.axis];
   const int var = s.var >= 0 ? s.var : c->var[s.axis];
@@ -240,6 +275,11 @@ gpue_status gpue::launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void
   a.invN = s.invN;
   a.norm_in = c->results + 0;
   a.partials = c->partials;
+  if (s.flags & F_SCALE_PART) {
+    a.norm_nb = s.part_nb;
+    a.norm_scale = c->d[0] * c->d[1] * c->d[2];
+    a.norm_out = c->results + 0;
+  }
   int slot = -1;
   if (c->timing) {
     if (c->ev_used == TIMING_PAIRS) {
@@ -257,9 +297,8 @@ gpue_status gpue::launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2 * slot + 1], c->stream));
   // second stage of the fused reductions
   const bool ksum = s.mode == (s.row ? (int)RM_KSUM : (int)CM_KSUM);
-  if (ksum || (s.flags & F_NORM)) {
-    const int nb = s.row ? sweep_blocks(c->prec, true, n, var, c->dims[1] * c->dims[2], 1)
-                         : sweep_blocks(c->prec, false, n, var, c->dims[0], s.axis == 1 ? c->dims[2] : c->dims[1]);
+  if (ksum || ((s.flags & F_NORM) && !(s.flags & F_DEFER))) {
+    const int nb = sweep_nb(c, s);
     const double dV = c->d[0] * c->d[1] * c->d[2];
     e = ksum ? launch_finalize(c->partials, nb, 1, 0, s.invN, c->results + 1, c->stream)
              : launch_finalize(c->partials, nb, 1, 1, dV, c->results + 0, c->stream);
@@ -1018,6 +1057,9 @@ static gpue_status build_graph(gpue_ctx* c, int gstate) {
   std::vector<int> step_of;
   const double Dt = gstate == 0 ? c->p.gdt : c->p.dt;
   schedule_block(c, gstate, 0, c->graph_steps, sched, step_of);
+  if (gstate == 0)
+    for (size_t k = 0; k + 1 < sched.size(); ++k)
+      if (step_of[k + 1] == step_of[k] + 1) defer_norm(c, sched[k], sched[k + 1]);
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   gpue_status st = GPUE_OK;
   for (size_t k = 0; k < sched.size() && st == GPUE_OK; ++k) {
@@ -1057,28 +1099,32 @@ static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gsta
   const double Dt = gstate == 0 ? c->p.gdt : c->p.dt;
   int s;
   const int G = c->graph_steps;
+  gpue_status st0 = settle_deferred(c);
+  if (st0 != GPUE_OK) return st0;
+  // steady state: step s starts a replay of a captured block of G steps when
+  // no energy cadence falls inside it and the fields do not depend on time
+  // (the imaginary-time graph always rescales on entry: results[0] holds the
+  // pending norm or 1.0, and x*1.0 is exact)
+  auto uses_graph = [&](int s2) {
+    const int i2 = first_step + s2;
+    if (!(G > 0 && !c->timing && !c->v_time && !c->ramp_type && (i2 % 2) == 0 && nsteps - s2 >= G)) return false;
+    if (energy_every > 0) {
+      const int nxt = ((i2 + energy_every - 1) / energy_every) * energy_every;
+      if (nxt < i2 + G) return false;
+    }
+    return true;
+  };
   for (s = 0; s < nsteps; ++s) {
     const int i = first_step + s;
-    // steady state: replay a captured block of G steps when no energy cadence
-    // falls inside it and the fields do not depend on time
-    // (the imaginary-time graph always rescales on entry: results[0] holds the
-    // pending norm or 1.0, and x*1.0 is exact)
-    if (G > 0 && !c->timing && !c->v_time && !c->ramp_type && (i % 2) == 0 && nsteps - s >= G) {
-      bool cadence_inside = false;
-      if (energy_every > 0) {
-        const int nxt = ((i + energy_every - 1) / energy_every) * energy_every;
-        cadence_inside = nxt < i + G;
+    if (uses_graph(s)) {
+      if (!c->gexec[gstate]) {
+        gpue_status st = build_graph(c, gstate);
+        if (st != GPUE_OK) return st;
       }
-      if (!cadence_inside) {
-        if (!c->gexec[gstate]) {
-          gpue_status st = build_graph(c, gstate);
-          if (st != GPUE_OK) return st;
-        }
-        HIPCHK(c, hipGraphLaunch(c->gexec[gstate], c->stream));
-        if (gstate == 0) c->pending_norm = true;
-        s += G - 1;
-        continue;
-      }
+      HIPCHK(c, hipGraphLaunch(c->gexec[gstate], c->stream));
+      if (gstate == 0) c->pending_norm = true;
+      s += G - 1;
+      continue;
     }
     Phys ph = make_phys(c, gstate, Dt * i);   // time = Dt*i (src/evolution.cu:408)
     if (c->ramp_type) ph.Dt = Dt * ramp_scale(c, i);
@@ -1087,6 +1133,25 @@ static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gsta
       if (st != GPUE_OK) return st;
     }
     schedule(c, gstate, i, c->pending_norm, sched);
+    if (c->defer_nb > 0) {   // the previous step left its norm partials to this one
+      if (sched[0].row && sched[0].mode == RM_FWD && (sched[0].flags & F_SCALE_IN)) {
+        sched[0].flags |= F_SCALE_PART;
+        sched[0].part_nb = c->defer_nb;
+        c->defer_nb = 0;
+      } else {
+        gpue_status st = settle_deferred(c);
+        if (st != GPUE_OK) return st;
+      }
+    }
+    // leave this step's norm to the next one when that is an eager step too
+    if (gstate == 0 && c->p.dimnum >= 2 && s + 1 < nsteps && !uses_graph(s + 1) &&
+        !(energy_every > 0 && i % energy_every == 0) && (sched.back().flags & F_NORM)) {
+      const int nb = sweep_nb(c, sched.back());
+      if (nb < 4096) {
+        sched.back().flags |= F_DEFER;
+        c->defer_nb = nb;
+      }
+    }
     for (const Sweep& sw : sched) {
       gpue_status st = launch(c, sw, ph, c->psi, c->psi);
       if (st != GPUE_OK) return st;
@@ -1108,6 +1173,10 @@ static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gsta
         break;
       }
     }
+  }
+  {
+    gpue_status st = settle_deferred(c);   // (no step defers past the last one)
+    if (st != GPUE_OK) return st;
   }
   r.steps_done = s;
   c->next_step = first_step + s;

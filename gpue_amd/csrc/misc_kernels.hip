@@ -81,24 +81,11 @@ __global__ void k_norm_sum(const cx<T>* psi, long N, double* partials) {
 }
 
 // Second stage of every reduction: one workgroup sums nb partials of each of
-// na accumulators in a fixed order.  mode 0: out[a] = tot[a] * scale;
-// mode 1: out[0] = sqrt(tot[0] * scale)  (the parSum norm, src/kernels.cu:482).
-// Sum of nb partials in a fixed order: lane t adds elements t, t+B, t+2B, ...
-// in four interleaved accumulators (so four loads are in flight per lane),
-// then the block tree.  The order depends only on nb and the block size.
+// na accumulators in a fixed order (fin_sum_at, then block_sum).  mode 0:
+// out[a] = tot[a] * scale; mode 1: out[0] = sqrt(tot[0] * scale) (the parSum
+// norm, src/kernels.cu:482).
 __device__ __forceinline__ double fin_sum(const double* __restrict__ p, int nb) {
-  const int B = blockDim.x;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int i = threadIdx.x;
-  for (; i + 3 * B < nb; i += 4 * B) {
-    const double a = p[i], b = p[i + B], c = p[i + 2 * B], d = p[i + 3 * B];
-    s0 += a;
-    s1 += b;
-    s2 += c;
-    s3 += d;
-  }
-  for (; i < nb; i += B) s0 += p[i];
-  return (s0 + s1) + (s2 + s3);
+  return fin_sum_at(p, nb, threadIdx.x, blockDim.x);
 }
 
 __global__ void __launch_bounds__(1024) k_finalize(const double* partials, int nb, int na, int mode, double scale,
@@ -321,12 +308,17 @@ hipError_t launch_norm_sum(const void* psi, long N, double* partials, int* nb, h
   return hipGetLastError();
 }
 
-__global__ void __launch_bounds__(256) k_finalize_b(const FinArgs* __restrict__ args) {
+// Batched finalize: one 1024-lane block per instance, each summing in the
+// order k_finalize uses for its nb (lanes past fin_block(nb) add nothing, and
+// the zero wave totals they leave are exact), so an instance's results are
+// the same batched or alone.
+__global__ void __launch_bounds__(1024) k_finalize_b(const FinArgs* __restrict__ args) {
   const FinArgs& f = args[blockIdx.x];
   if (!f.active) return;
   __shared__ double sh[64];
+  const int B = fin_block(f.nb);
   for (int a = 0; a < f.na; ++a) {
-    double acc[1] = {fin_sum(f.partials + (size_t)a * f.nb, f.nb)};
+    double acc[1] = {(int)threadIdx.x < B ? fin_sum_at(f.partials + (size_t)a * f.nb, f.nb, threadIdx.x, B) : 0.0};
     block_sum<1>(acc, sh);
     if (threadIdx.x == 0) f.out[a] = f.mode == 1 ? sqrt(acc[0] * f.scale) : acc[0] * f.scale;
     __syncthreads();
@@ -334,12 +326,12 @@ __global__ void __launch_bounds__(256) k_finalize_b(const FinArgs* __restrict__ 
 }
 
 hipError_t launch_finalize_batched(const FinArgs* d, int ninst, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize_b, dim3(ninst), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_finalize_b, dim3(ninst), dim3(1024), 0, s, d);
   return hipGetLastError();
 }
 
 hipError_t launch_finalize(const double* partials, int nb, int na, int mode, double scale, double* out, hipStream_t s) {
-  const int bs = nb >= 4096 ? 1024 : 256;   // block_sum takes up to 16 waves
+  const int bs = fin_block(nb);   // block_sum takes up to 16 waves
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(bs), 0, s, partials, nb, na, mode, scale, out);
   return hipGetLastError();
 }

@@ -103,7 +103,17 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
     }
     if constexpr (MODE == RM_FWD || MODE == RM_FULL) {
       if (flags & F_SCALE_IN) {
-        const T inv = (T)(1.0 / *a.norm_in);
+        // a deferred renormalisation: the norm reduced here from the previous
+        // sweep's partials by every wave (all lanes active; psi is already in
+        // flight), stored once by block 0
+        double nrm;
+        if (MODE == RM_FWD && (flags & F_SCALE_PART)) {
+          nrm = part_norm(a.partials, a.norm_nb, a.norm_scale);
+          if (bid == 0 && threadIdx.x == 0) *a.norm_out = nrm;
+        } else {
+          nrm = *a.norm_in;
+        }
+        const T inv = (T)(1.0 / nrm);
 #pragma unroll
         for (int r = 0; r < E; ++r) v[r] = scale_norm(v[r], inv);
       }

@@ -37,6 +37,11 @@ enum : unsigned {
   F_HALFV_POST = 4u,   // V/2 (+ nonlinearity) after the inverse transform
   F_GAUGE_POST = 8u,   // extra [FFT, A_axis, IFFT] after the post op
   F_NORM = 16u,        // accumulate |psi_out|^2 and produce norm_out
+  F_SCALE_PART = 32u,  // with F_SCALE_IN: the norm is reduced here from the previous
+                       // sweep's partials (norm_nb of them), and block 0 stores it
+                       // to norm_out, instead of a separate finalize
+  F_DEFER = 64u,       // host only: the finalize of this F_NORM sweep is left to
+                       // the next sweep (F_SCALE_PART)
 };
 
 // Time-dependent equation program (postfix): CONST pushes val, X/Y/Z/T push a
@@ -83,6 +88,9 @@ struct SweepArgs {
   double invN;        // normalisation folded into K (1/N) or A (1/n_axis)
   const double* norm_in;   // pending renormalisation divisor (F_SCALE_IN)
   double* partials;        // per-workgroup partial sums (F_NORM, KSUM modes)
+  int norm_nb;             // F_SCALE_PART: partials of the previous sweep
+  double norm_scale;       // F_SCALE_PART: dV
+  double* norm_out;        // F_SCALE_PART: where block 0 stores the norm
 };
 
 // ---------------------------------------------------------------------------
@@ -512,6 +520,87 @@ __device__ __forceinline__ void block_sum(double (&v)[NA], double* sh /* >= NA *
       v[a] = s;
     }
   }
+}
+
+// Sum of nb partials in a fixed order, as lane t of a B-lane finalize block
+// adds them: elements t, t+B, t+2B, ... in four interleaved accumulators (so
+// four loads are in flight per lane).  The order depends only on nb and B.
+__device__ __forceinline__ double fin_sum_at(const double* __restrict__ p, int nb, int t, int B) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int i = t;
+  for (; i + 3 * B < nb; i += 4 * B) {
+    const double a = p[i], b = p[i + B], c = p[i + 2 * B], d = p[i + 3 * B];
+    s0 += a;
+    s1 += b;
+    s2 += c;
+    s3 += d;
+  }
+  for (; i < nb; i += B) s0 += p[i];
+  return (s0 + s1) + (s2 + s3);
+}
+
+// The finalize block size for nb partials (launch_finalize).
+__host__ __device__ constexpr int fin_block(int nb) { return nb >= 4096 ? 1024 : 256; }
+
+// fin_sum_at(p, nb, t, 256) for the four lanes t = l + 64j (j < 4) of one
+// finalize block, nb <= 256*KM: all 4*KM loads first (zero past nb), then the
+// same additions in the same order (adding an exact zero changes nothing).
+template <int KM>
+__device__ __forceinline__ void fin_sum4(const double* __restrict__ p, int nb, int l, double (&out)[4]) {
+  constexpr int B = 256;
+  double v[4][KM];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const int i = l + 64 * j + k * B;
+      v[j][k] = i < nb ? p[i] : 0.0;
+    }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int t = l + 64 * j;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    bool grouped = true;   // fin_sum_at's first loop: groups of four while t + (4g+3)B < nb
+#pragma unroll
+    for (int g = 0; g < KM / 4; ++g) {
+      grouped = grouped && (t + (4 * g + 3) * B < nb);
+      s0 += grouped ? v[j][4 * g] : 0.0;
+      s1 += grouped ? v[j][4 * g + 1] : 0.0;
+      s2 += grouped ? v[j][4 * g + 2] : 0.0;
+      s3 += grouped ? v[j][4 * g + 3] : 0.0;
+    }
+    // its second loop: the rest, one by one into s0 (the grouped ones were
+    // taken above: skip them)
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const bool taken = (k / 4 < KM / 4) && (t + (4 * (k / 4) + 3) * B < nb);
+      s0 += taken ? 0.0 : v[j][k];
+    }
+    out[j] = (s0 + s1) + (s2 + s3);
+  }
+}
+
+// The parSum norm sqrt(sum * scale) of nb < 4096 partials, computed by one full
+// wave (every lane gets it) with the additions of k_finalize in its 256-lane
+// form, bit for bit: lane l stands for finalize lanes l + 64j (j < 4); each j
+// is one finalize wave, reduced by the same xor butterfly as block_sum (the
+// four run interleaved), and the four wave totals are added in order.
+__device__ __forceinline__ double part_norm(const double* __restrict__ p, int nb, double scale) {
+  const int l = threadIdx.x & 63;
+  double x[4];
+  if (nb <= 256) fin_sum4<1>(p, nb, l, x);
+  else if (nb <= 512) fin_sum4<2>(p, nb, l, x);
+  else if (nb <= 1024) fin_sum4<4>(p, nb, l, x);
+  else if (nb <= 2048) fin_sum4<8>(p, nb, l, x);
+  else fin_sum4<16>(p, nb, l, x);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) x[j] += __shfl_xor(x[j], o, 64);
+  double tot = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) tot += x[j];
+  return sqrt(tot * scale);
 }
 
 // Block b of nb (of one problem) stores its NA partial sums.
