@@ -78,9 +78,6 @@ __device__ __forceinline__ cx<float> cmul_split(cx<float> a, float hx, float hy,
   return mk(fmaf(a.x, hx, re), fmaf(a.x, hy, im));
 }
 
-#ifndef GPUE_SPLIT_DFT
-#define GPUE_SPLIT_DFT 1
-#endif
 // multiply a by exp(SIGN*2*pi*i*m/R), with the exact special cases folded
 template <typename T, int R, int SIGN>
 __device__ __forceinline__ cx<T> twc(cx<T> a, int m) {
@@ -91,7 +88,7 @@ __device__ __forceinline__ cx<T> twc(cx<T> a, int m) {
   if (4 * m == 3 * R) return mul_si<T, -SIGN>(a);
   const int m48 = m * (48 / R);
   const double cd = Tw48::cosv(m48), sd = SIGN * Tw48::sinv(m48);
-  if constexpr (sizeof(T) == 4 && GPUE_SPLIT_DFT) {
+  if constexpr (sizeof(T) == 4) {
     if ((8 * m) % R == 0) {
       // odd multiple of 45 degrees: w = c*(sx + i*sy) with c = sqrt(1/2) split,
       // sx, sy = +-1: one exact sign pattern, then two split scalings
