@@ -10,9 +10,11 @@
 // [FWD][K][INV][A_y][A_x+norm]) both parities share one launch per position,
 // with per-instance flags, and the extra position is inactive for odd
 // instances; otherwise each parity has its own sequence.  Every imaginary-time
-// sequence ends with one batched norm finalize.  Two consecutive steps of all
-// groups are captured once per gstate as a HIP graph and replayed.  Device
-// memory is allocated only in gpue_batch_create.
+// sequence ends with one batched norm finalize.  Each group runs on its own
+// stream, with two consecutive steps captured once per gstate as a HIP graph
+// and replayed, so the groups of different shapes (the small ones latency
+// bound) overlap on the device.  Device memory is allocated only in
+// gpue_batch_create.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -38,6 +40,9 @@ struct Group {
   gpue_ctx* proto;                  // first member: shape and variants
   std::vector<Sweep> sched[2][2];   // [gstate][parity]
   std::vector<BSeq> seqs[2][2];     // [gstate][phase]
+  hipStream_t stream = nullptr;     // the group's own stream (default priority)
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
 };
 
 bool same_shape(const gpue_ctx* a, const gpue_ctx* b) {
@@ -60,14 +65,11 @@ bool mergeable(const std::vector<Sweep>& a, const std::vector<Sweep>& b) {
 
 struct gpue_batch {
   int device = 0;
-  hipStream_t stream = nullptr;
   std::vector<gpue_ctx*> ctx;
   std::vector<int> active;
   std::vector<int> cls;            // parity class: the instance's step parity at phase 0
   std::vector<Group> groups;
   int phase = 0;
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::string err;
 };
 
@@ -149,9 +151,9 @@ static gpue_status upload_args(gpue_batch* b) {
   return GPUE_OK;
 }
 
-// One step (one phase) of all groups for gstate.
-static gpue_status launch_phase(gpue_batch* b, int gs, int ph) {
-  for (Group& g : b->groups) {
+// One step (one phase) of group g for gstate, on the group's stream.
+static gpue_status launch_group_phase(gpue_batch* b, Group& g, int gs, int ph) {
+  {
     gpue_ctx* c = g.proto;
     for (BSeq& s : g.seqs[gs][ph]) {
       for (size_t k = 0; k < s.keys.size(); ++k) {
@@ -161,11 +163,11 @@ static gpue_status launch_phase(gpue_batch* b, int gs, int ph) {
         const int others = sw.row ? 1 : (sw.axis == 1 ? c->dims[2] : c->dims[1]);
         bool handled = false;
         hipError_t e = launch_sweep_batched(c->prec, sw.row, n, c->var[sw.axis], sw.mode, s.dargs[k],
-                                            (int)g.members.size(), lx, others, b->stream, &handled);
+                                            (int)g.members.size(), lx, others, g.stream, &handled);
         if (!handled || e != hipSuccess) return bfail(b, GPUE_EHIP, "batched sweep launch");
       }
       if (s.dfin) {
-        hipError_t e = launch_finalize_batched(s.dfin, (int)g.members.size(), b->stream);
+        hipError_t e = launch_finalize_batched(s.dfin, (int)g.members.size(), g.stream);
         if (e != hipSuccess) return bfail(b, GPUE_EHIP, "batched finalize launch");
       }
     }
@@ -173,20 +175,36 @@ static gpue_status launch_phase(gpue_batch* b, int gs, int ph) {
   return GPUE_OK;
 }
 
-static void drop_batch_graphs(gpue_batch* b) {
-  for (int g = 0; g < 2; ++g) {
-    if (b->gexec[g]) (void)hipGraphExecDestroy(b->gexec[g]);
-    if (b->graph[g]) (void)hipGraphDestroy(b->graph[g]);
-    b->gexec[g] = nullptr;
-    b->graph[g] = nullptr;
+static gpue_status launch_phase(gpue_batch* b, int gs, int ph) {
+  for (Group& g : b->groups) {
+    gpue_status st = launch_group_phase(b, g, gs, ph);
+    if (st != GPUE_OK) return st;
   }
+  return GPUE_OK;
+}
+
+static void drop_batch_graphs(gpue_batch* b) {
+  for (Group& grp : b->groups)
+    for (int g = 0; g < 2; ++g) {
+      if (grp.gexec[g]) (void)hipGraphExecDestroy(grp.gexec[g]);
+      if (grp.graph[g]) (void)hipGraphDestroy(grp.graph[g]);
+      grp.gexec[g] = nullptr;
+      grp.graph[g] = nullptr;
+    }
+}
+
+static gpue_status sync_groups(gpue_batch* b) {
+  for (Group& g : b->groups)
+    if (g.stream) BHIP(b, hipStreamSynchronize(g.stream));
+  return GPUE_OK;
 }
 
 extern "C" {
 
 void gpue_batch_destroy(gpue_batch* b) {
   if (!b) return;
-  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (Group& g : b->groups)
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
   drop_batch_graphs(b);
   for (Group& g : b->groups)
     for (int gs = 0; gs < 2; ++gs)
@@ -196,7 +214,8 @@ void gpue_batch_destroy(gpue_batch* b) {
             if (p) (void)hipFree(p);
           if (s.dfin) (void)hipFree(s.dfin);
         }
-  if (b->stream) (void)hipStreamDestroy(b->stream);
+  for (Group& g : b->groups)
+    if (g.stream) (void)hipStreamDestroy(g.stream);
   delete b;
 }
 
@@ -217,8 +236,7 @@ gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
     b->cls.push_back(c->next_step % 2);
   }
   b->device = ctxs[0]->p.device;
-  if (hipSetDevice(b->device) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipSetDevice(b->device) != hipSuccess) {
     gpue_batch_destroy(b);
     return GPUE_EHIP;
   }
@@ -237,6 +255,11 @@ gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
       b->groups.push_back(g);
     }
   }
+  for (Group& g : b->groups)
+    if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess) {
+      gpue_batch_destroy(b);
+      return GPUE_EHIP;
+    }
   for (Group& g : b->groups)
     for (int gs = 0; gs < 2; ++gs) {
       // imaginary time: the norm slot holds 1.0 whenever nothing is pending, so
@@ -323,24 +346,27 @@ gpue_status gpue_batch_evolve(gpue_batch* b, int nsteps, int gstate) {
     ++s;
   }
   if (nsteps - s >= 2) {
-    if (!b->gexec[gstate]) {
-      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      gpue_status st = launch_phase(b, gstate, 0);
-      if (st == GPUE_OK) st = launch_phase(b, gstate, 1);
+    for (Group& grp : b->groups) {
+      if (grp.gexec[gstate]) continue;
+      BHIP(b, hipStreamBeginCapture(grp.stream, hipStreamCaptureModeThreadLocal));
+      gpue_status st = launch_group_phase(b, grp, gstate, 0);
+      if (st == GPUE_OK) st = launch_group_phase(b, grp, gstate, 1);
       hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      hipError_t e = hipStreamEndCapture(grp.stream, &g);
       if (st != GPUE_OK) {
         if (g) (void)hipGraphDestroy(g);
         return st;
       }
       if (e != hipSuccess) return bfail(b, GPUE_EHIP, "batch graph capture");
-      if (hipGraphInstantiate(&b->gexec[gstate], g, nullptr, nullptr, 0) != hipSuccess) {
+      if (hipGraphInstantiate(&grp.gexec[gstate], g, nullptr, nullptr, 0) != hipSuccess) {
         (void)hipGraphDestroy(g);
         return bfail(b, GPUE_EHIP, "batch graph instantiate");
       }
-      b->graph[gstate] = g;
+      grp.graph[gstate] = g;
     }
-    for (; nsteps - s >= 2; s += 2) BHIP(b, hipGraphLaunch(b->gexec[gstate], b->stream));
+    // pairs of steps, the groups' graphs interleaved across their streams
+    for (; nsteps - s >= 2; s += 2)
+      for (Group& grp : b->groups) BHIP(b, hipGraphLaunch(grp.gexec[gstate], grp.stream));
   }
   if (s < nsteps) {
     gpue_status st = launch_phase(b, gstate, 0);
@@ -348,7 +374,10 @@ gpue_status gpue_batch_evolve(gpue_batch* b, int nsteps, int gstate) {
     b->phase = 1;
     ++s;
   }
-  BHIP(b, hipStreamSynchronize(b->stream));
+  {
+    gpue_status st = sync_groups(b);
+    if (st != GPUE_OK) return st;
+  }
   for (size_t i = 0; i < b->ctx.size(); ++i) {
     if (!b->active[i]) continue;
     b->ctx[i]->next_step += nsteps;

@@ -185,8 +185,11 @@ double      gpue_eqn_eval(const gpue_eqn_node *nodes, int n_nodes, double x, dou
 
 /* --- batched instances (no reference counterpart: GPUE solves one problem per
  * process; its bin/run.sh starts several processes).  Instances of one shape
- * share every sweep launch; two steps of all instances are captured as a HIP
- * graph and replayed.  Contexts stay owned by the caller and must outlive the
+ * share every sweep launch; each shape group runs on its own stream, two steps
+ * captured as a HIP graph and replayed, so the groups overlap on the device.
+ * The kernel variants are the contexts' own (equal within a group; a batch
+ * may prefer other variants than a lone instance: the groups share the
+ * device).  Contexts stay owned by the caller and must outlive the
  * batch; results are read through the contexts (gpue_energy, gpue_download_wfc).
  * Contexts with a time-dependent V or on different devices are rejected. --- */
 typedef struct gpue_batch gpue_batch;
