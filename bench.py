@@ -332,7 +332,7 @@ def main():
                     help="kernel variants per axis, e.g. 1,1,1 (default: autotune)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; every solve runs on a single MI355X "
+        print("bench.py measures one GPU: every solve runs on a single MI355X "
               "(multi-GPU sharding is a design note only). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 

@@ -106,7 +106,7 @@ Grid parse_args(int argc, char** argv) {
   par.store("use_param_file", false);
   par.store("param_file", std::string("param.cfg"));
   par.store("thresh_const", 1.0);
-  // extensions of this build: explicit hbar/mass/g for oscillator units, the
+  // extensions beyond GPUE's flags: explicit hbar/mass/g for oscillator units, the
   // working precision, and the imaginary->real hand-off policy
   par.store("hbar", HBAR_SI);
   par.store("mass", 1.4431607e-25);   // src/init.cu:98
