@@ -139,6 +139,8 @@ def kernel_table(ctx, prec, variants, steps_by_gstate, reps=50):
 
 
 def roofline(rows, bytes_per_launch, workload):
+    if not rows:   # --no-profile
+        return None
     top = rows[0]
     achieved = bytes_per_launch / (top["us_per_launch"] * 1e-6) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -318,6 +320,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-extra", action="store_true", help="skip the 256^3 / latency extras")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="skip the per-kernel graph replays and the roofline (for runs under a kernel tracer, "
+                         "whose graph interception has crashed on the host)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--only", choices=["512", "256c", "lat", "batch", "serve", "copy"], default=None,
                     help="profiling: run one workload only (copy: a plain 256 MiB device copy, the "
@@ -363,7 +368,7 @@ def main():
         bps_imag, P_imag = ctx.step_bytes(0)
         bps_real, P_real = ctx.step_bytes(1)
         bytes_job = imag * bps_imag + real * bps_real
-        rows, bpl = kernel_table(ctx, 64, variants, {0: imag, 1: real})
+        rows, bpl = kernel_table(ctx, 64, variants, {0: imag, 1: real}) if not args.no_profile else ([], 0)
         ctx.close()
         log(f"512^2 FP64 gauge: {sps:.1f} steps/s ({el / K * 1e6:.2f} us/step; imaginary "
             f"{(t_mid - t0) / max(imag, 1) * 1e6:.2f}, real {(t_end - t_mid) / max(real, 1) * 1e6:.2f}), "
@@ -409,7 +414,7 @@ def main():
             el3 = time.perf_counter() - t0
             e3, mu3 = ctx.energy()
             b3, P3 = ctx.step_bytes(0)
-            rows3, bpl3 = kernel_table(ctx, prec, variants, {0: K3}, reps=10)
+            rows3, bpl3 = kernel_table(ctx, prec, variants, {0: K3}, reps=10) if not args.no_profile else ([], 0)
             ctx.close()
             res3[prec] = dict(steps_per_s=round(K3 / el3, 2), us_per_step=round(el3 / K3 * 1e6, 2),
                               effective_GBps=round(b3 * K3 / el3 / 1e9, 1),
