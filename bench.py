@@ -454,8 +454,13 @@ def main():
         extra["batched_sweep_64_uneven_fp32"] = batched_sweep(G)
         log("batched sweep:", json.dumps(extra["batched_sweep_64_uneven_fp32"]))
     if not args.no_extra and args.only in (None, "serve"):
+        # a burst far above capacity (throughput, queueing latency), then the
+        # same mix offered at about half the measured capacity (service latency)
         extra["serving_stream"] = serving_stream(G)
         log("serving stream:", json.dumps(extra["serving_stream"]))
+        light = round(0.5 * extra["serving_stream"]["throughput_req_s"], 1)
+        extra["serving_stream_half_load"] = serving_stream(G, rate=light)
+        log("serving stream at half load:", json.dumps(extra["serving_stream_half_load"]))
 
     cpu = None
     if not args.no_cpu and args.only is None:
