@@ -409,4 +409,27 @@ __device__ __forceinline__ void fft_regs(cx<T>* v, int tid, const typename TwE<T
   fft_from_pass<T, N, V, ORD, SIGN, PRE, 0>(v, tid, tw, w, lds, la, active);
 }
 
+// The twiddles of both orders, loaded once at the start of a sweep (PRE
+// variants): every transform of the sweep then runs on registers, so a second
+// or third transform does not wait for its table reads.
+template <typename T, int N, int V>
+struct TwRegs {
+  typename TwE<T>::type f[TwPlan<N, V, ORD_FWD>::total()];
+  typename TwE<T>::type r[TwPlan<N, V, ORD_REV>::total()];
+  __device__ __forceinline__ void load(int tid, const typename TwE<T>::type* __restrict__ tw) {
+    tw_load<T, N, V, ORD_FWD>(tid, tw, f);
+    tw_load<T, N, V, ORD_REV>(tid, tw, r);
+  }
+};
+
+// fft_regs on preloaded twiddles (PRE) or from the table.
+template <typename T, int N, int V, int ORD, int SIGN, bool PRE, typename LA>
+__device__ __forceinline__ void fft_regs_tw(cx<T>* v, int tid, const typename TwE<T>::type* __restrict__ tw,
+                                            const TwRegs<T, N, V>& wr, cx<T>* lds, const LA& la, bool active) {
+  if constexpr (PRE)
+    fft_from_pass<T, N, V, ORD, SIGN, true, 0>(v, tid, tw, ORD == ORD_FWD ? wr.f : wr.r, lds, la, active);
+  else
+    fft_from_pass<T, N, V, ORD, SIGN, false, 0>(v, tid, tw, wr.f, lds, la, active);
+}
+
 }  // namespace gpue

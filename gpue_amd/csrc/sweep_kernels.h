@@ -80,6 +80,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
   constexpr int RF0 = Seq<N, V, ORD_FWD>::first(), RFL = Seq<N, V, ORD_FWD>::last();
   cx<T> v[E];
   double acc = 0.0;
+  TwRegs<T, N, V> twr;   // PRE: every twiddle of the sweep, loaded after psi
   // grid indices of register slot r in the boundary layouts of the first / last radix
   auto pos0 = [&](int r) { return Pt{bpos<N, TPL, RF0>(tid, r), yi, zi}; };
   auto posL = [&](int r) { return Pt{bpos<N, TPL, RFL>(tid, r), yi, zi}; };
@@ -89,18 +90,20 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 #pragma unroll
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(tid, r)];
     }
-    fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
+    if constexpr (PRE) twr.load(tid, tw);
+    fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     if (flags & F_HALFV_POST) halfV_all<T, E>(p, v, pos0);
     if (flags & F_GAUGE_POST) {
-      fft_regs<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, lds, la, active);
+      fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
       gauge_all<T, E, 0>(p, v, posL, a.invN);
-      fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
+      fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     }
   } else {
     if (active) {
 #pragma unroll
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RF0>(tid, r)];
     }
+    if constexpr (PRE) twr.load(tid, tw);
     if constexpr (MODE == RM_FWD || MODE == RM_FULL) {
       if (flags & F_SCALE_IN) {
         // a deferred renormalisation: the norm reduced here from the previous
@@ -119,18 +122,18 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
       }
       if (flags & F_HALFV_PRE) halfV_all<T, E>(p, v, pos0);
     }
-    fft_regs<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, lds, la, active);
+    fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
     if constexpr (MODE == RM_GAUGE || MODE == RM_GAUGE_FWD) {
       gauge_all<T, E, 0>(p, v, posL, a.invN);
-      fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
+      fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     } else if constexpr (MODE == RM_FULL) {
       kstep_all<T, E>(p, v, posL, a.invN);
-      fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
+      fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
       if (flags & F_HALFV_POST) halfV_all<T, E>(p, v, pos0);
       if (flags & F_GAUGE_POST) {
-        fft_regs<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, lds, la, active);
+        fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
         gauge_all<T, E, 0>(p, v, posL, a.invN);
-        fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
+        fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
       }
     } else if constexpr (MODE == RM_KSUM) {
       if (active) {
@@ -146,7 +149,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
   // bridge modes continue with the next step's first row sweep
   if constexpr (MODE == RM_INV_FWD || MODE == RM_GAUGE_FWD) {
     if (flags & F_HALFV_PRE) halfV_all<T, E>(p, v, pos0);
-    fft_regs<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, lds, la, active);
+    fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
   }
 
   // store: forward sweeps leave the forward layout (last radix), the others the first radix
@@ -199,6 +202,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
   constexpr int RF0 = Seq<N, V, ORD_FWD>::first(), RFL = Seq<N, V, ORD_FWD>::last();
   cx<T> v[E];
   double acc = 0.0;
+  TwRegs<T, N, V> twr;   // PRE: every twiddle of the sweep, loaded after psi
   // coordinates of element e along the axis
   auto yz = [&](int e, int& yi, int& zi) {
     if (a.axis == 1) { yi = e; zi = o; } else { yi = o; zi = e; }
@@ -209,7 +213,8 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
 #pragma unroll
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(tid, r) * es];
     }
-    fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
+    if constexpr (PRE) twr.load(tid, tw);
+    fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     if (active) {
 #pragma unroll
       for (int r = 0; r < E; ++r) out[bpos<N, TPL, RF0>(tid, r) * es] = v[r];
@@ -219,7 +224,8 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
 #pragma unroll
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RF0>(tid, r) * es];
     }
-    fft_regs<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, lds, la, active);
+    if constexpr (PRE) twr.load(tid, tw);
+    fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
     if constexpr (MODE == CM_FWD) {
       if (active) {
 #pragma unroll
@@ -247,7 +253,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
         if (a.axis == 1) gauge_all<T, E, 1>(p, v, posL, a.invN);
         else gauge_all<T, E, 2>(p, v, posL, a.invN);
       }
-      fft_regs<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, lds, la, active);
+      fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
       if (active) {
 #pragma unroll
         for (int r = 0; r < E; ++r) {

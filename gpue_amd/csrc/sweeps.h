@@ -543,41 +543,38 @@ __device__ __forceinline__ double fin_sum_at(const double* __restrict__ p, int n
 __host__ __device__ constexpr int fin_block(int nb) { return nb >= 4096 ? 1024 : 256; }
 
 // fin_sum_at(p, nb, t, 256) for the four lanes t = l + 64j (j < 4) of one
-// finalize block, nb <= 256*KM: all 4*KM loads first (zero past nb), then the
-// same additions in the same order (adding an exact zero changes nothing).
-template <int KM>
+// finalize block: in chunks of four consecutive elements per lane (k = 4c ..
+// 4c+3, element t + k*256; zero past nb), all sixteen loads of a chunk first,
+// then the same additions in the same order: a chunk is one iteration of the
+// first loop while t + (4c+3)*256 < nb (monotone in c), else its elements go
+// one by one into s0 as the second loop takes them (adding an exact zero
+// changes nothing).
 __device__ __forceinline__ void fin_sum4(const double* __restrict__ p, int nb, int l, double (&out)[4]) {
   constexpr int B = 256;
-  double v[4][KM];
+  double s[4][4] = {};
+  const int nch = (nb + 4 * B - 1) / (4 * B);
+  for (int c = 0; c < nch; ++c) {
+    double v[4][4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      const int i = l + 64 * j + k * B;
-      v[j][k] = i < nb ? p[i] : 0.0;
+      for (int q = 0; q < 4; ++q) {
+        const int i = l + 64 * j + (4 * c + q) * B;
+        v[j][q] = i < nb ? p[i] : 0.0;
+      }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (l + 64 * j + (4 * c + 3) * B < nb) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[j][q] += v[j][q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[j][0] += v[j][q];
+      }
     }
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int t = l + 64 * j;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    bool grouped = true;   // fin_sum_at's first loop: groups of four while t + (4g+3)B < nb
-#pragma unroll
-    for (int g = 0; g < KM / 4; ++g) {
-      grouped = grouped && (t + (4 * g + 3) * B < nb);
-      s0 += grouped ? v[j][4 * g] : 0.0;
-      s1 += grouped ? v[j][4 * g + 1] : 0.0;
-      s2 += grouped ? v[j][4 * g + 2] : 0.0;
-      s3 += grouped ? v[j][4 * g + 3] : 0.0;
-    }
-    // its second loop: the rest, one by one into s0 (the grouped ones were
-    // taken above: skip them)
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      const bool taken = (k / 4 < KM / 4) && (t + (4 * (k / 4) + 3) * B < nb);
-      s0 += taken ? 0.0 : v[j][k];
-    }
-    out[j] = (s0 + s1) + (s2 + s3);
   }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) out[j] = (s[j][0] + s[j][1]) + (s[j][2] + s[j][3]);
 }
 
 // The parSum norm sqrt(sum * scale) of nb < 4096 partials, computed by one full
@@ -588,11 +585,7 @@ __device__ __forceinline__ void fin_sum4(const double* __restrict__ p, int nb, i
 __device__ __forceinline__ double part_norm(const double* __restrict__ p, int nb, double scale) {
   const int l = threadIdx.x & 63;
   double x[4];
-  if (nb <= 256) fin_sum4<1>(p, nb, l, x);
-  else if (nb <= 512) fin_sum4<2>(p, nb, l, x);
-  else if (nb <= 1024) fin_sum4<4>(p, nb, l, x);
-  else if (nb <= 2048) fin_sum4<8>(p, nb, l, x);
-  else fin_sum4<16>(p, nb, l, x);
+  fin_sum4(p, nb, l, x);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
