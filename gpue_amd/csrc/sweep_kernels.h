@@ -192,6 +192,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
   const int xt = bid % tiles, o = bid / tiles;
   const int xi = xt * TC + c;
   const bool active = xi < p.xDim;
+  const int xq = active ? xi : 0;   // table index: lanes past xDim (a last partial tile) read column 0
   const long es = (a.axis == 1) ? (long)p.xDim : (long)p.xDim * p.yDim;
   const long base = (a.axis == 1) ? (long)(active ? xi : 0) + (long)o * p.xDim * p.yDim
                                   : (long)(active ? xi : 0) + (long)o * p.xDim;
@@ -225,6 +226,11 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RF0>(tid, r) * es];
     }
     if constexpr (PRE) twr.load(tid, tw);
+    auto posL = [&](int r) {
+      const int e = bpos<N, TPL, RFL>(tid, r);
+      return a.axis == 1 ? Pt{xq, e, o} : Pt{xq, o, e};
+    };
+    auto epos = [&](int r) { return bpos<N, TPL, RFL>(tid, r); };
     fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
     if constexpr (MODE == CM_FWD) {
       if (active) {
@@ -241,14 +247,9 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
         }
       }
     } else {
-      auto posL = [&](int r) {
-        const int e = bpos<N, TPL, RFL>(tid, r);
-        return a.axis == 1 ? Pt{xi, e, o} : Pt{xi, o, e};
-      };
       if constexpr (MODE == CM_K) {
-        auto epos = [&](int r) { return bpos<N, TPL, RFL>(tid, r); };
-        if (a.axis == 1) kstep_col<T, E, 1>(p, v, xi, o, epos, a.invN);
-        else kstep_col<T, E, 2>(p, v, xi, o, epos, a.invN);
+        if (a.axis == 1) kstep_col<T, E, 1>(p, v, xq, o, epos, a.invN);
+        else kstep_col<T, E, 2>(p, v, xq, o, epos, a.invN);
       } else {
         if (a.axis == 1) gauge_all<T, E, 1>(p, v, posL, a.invN);
         else gauge_all<T, E, 2>(p, v, posL, a.invN);

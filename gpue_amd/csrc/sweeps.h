@@ -333,9 +333,10 @@ __device__ __forceinline__ void apply_phase_all(cx<T>* v, const T (&arg)[E], int
 
 // V/2 with the nonlinear density term: cMultDensity (src/kernels.cu:365-388)
 // with GV/EV (src/operators.cu:1081,1100) merged into one exponential.
+// In two parts: halfV_vargs gathers V*cV of the lane's points, halfV_apply
+// adds the density term and applies the phase (halfV_all does both in a row).
 template <typename T, int E, typename F>
-__device__ __forceinline__ void halfV_all(const Phys& p, cx<T>* v, F pos) {
-  T arg[E];
+__device__ __forceinline__ void halfV_vargs(const Phys& p, F pos, T (&arg)[E]) {
   if (p.v_kind == V_SEP) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
@@ -355,6 +356,10 @@ __device__ __forceinline__ void halfV_all(const Phys& p, cx<T>* v, F pos) {
   const T cV = (T)p.cV;
 #pragma unroll
   for (int r = 0; r < E; ++r) arg[r] = arg[r] * cV;
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void halfV_apply(const Phys& p, cx<T>* v, T (&arg)[E]) {
   if (p.gpe) {
     const T g = (T)p.g, cH = (T)p.cH;
 #pragma unroll
@@ -366,11 +371,18 @@ __device__ __forceinline__ void halfV_all(const Phys& p, cx<T>* v, F pos) {
   apply_phase_all<T, E>(v, arg, p.gstate, 1.0);
 }
 
-// gauge operator along AXIS: GpA*/EpA* (src/operators.cu:1089-1110) with both
-// 1/sqrt(n) of the axis transforms folded into invn.
-template <typename T, int E, int AXIS, typename F>
-__device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double invn) {
+template <typename T, int E, typename F>
+__device__ __forceinline__ void halfV_all(const Phys& p, cx<T>* v, F pos) {
   T arg[E];
+  halfV_vargs<T, E>(p, pos, arg);
+  halfV_apply<T, E>(p, v, arg);
+}
+
+// gauge operator along AXIS: GpA*/EpA* (src/operators.cu:1089-1110) with both
+// 1/sqrt(n) of the axis transforms folded into invn.  gauge_args gathers the
+// exponents A*k*Dt of the lane's points; apply_phase_all applies the phase.
+template <typename T, int E, int AXIS, typename F>
+__device__ __forceinline__ void gauge_args(const Phys& p, F pos, T (&arg)[E]) {
   if (p.a_kind == A_TABLE) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
@@ -396,6 +408,12 @@ __device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double
     const T k = (T)(AXIS == 0 ? p.pxg[q.x] : AXIS == 1 ? p.pyg[q.y] : p.pzg[q.z]);
     arg[r] = arg[r] * k * Dt;
   }
+}
+
+template <typename T, int E, int AXIS, typename F>
+__device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double invn) {
+  T arg[E];
+  gauge_args<T, E, AXIS>(p, pos, arg);
   apply_phase_all<T, E>(v, arg, p.gstate, invn);
 }
 
@@ -450,8 +468,49 @@ __device__ __forceinline__ void kstep_all(const Phys& p, cx<T>* v, F pos, double
 }
 
 // K step on a strided line (axis AXIS = 1 or 2 varies along the line; x = xi
-// and the other coordinate o are fixed): the fixed axes' factors are loaded
-// once, only the varying axis' factor per point.  Same products as kstep_all.
+// and the other coordinate o are fixed), separable K: the fixed axes' factors
+// are loaded once, only the varying axis' factor per point.  Same products as
+// kstep_all.
+template <typename T, int E, int AXIS>
+struct KCol {
+  T fx0, fx1, fo0, fo1, fe[E][2];
+  template <typename F>
+  __device__ __forceinline__ void load(const Phys& p, int xi, int o, F epos) {
+    const auto* fxp = KTab<T>::get(p, 0) + 2 * xi;
+    fx0 = (T)fxp[0];
+    fx1 = (T)fxp[1];
+    const auto* fop = KTab<T>::get(p, AXIS == 1 ? 2 : 1) + 2 * o;
+    fo0 = (T)fop[0];
+    fo1 = (T)fop[1];
+    const auto* fvp = KTab<T>::get(p, AXIS);
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const int e = epos(r);
+      fe[r][0] = (T)fvp[2 * e];
+      fe[r][1] = (T)fvp[2 * e + 1];
+    }
+  }
+  __device__ __forceinline__ void apply(const Phys& p, cx<T>* v, double invN_) const {
+    const T invN = (T)invN_;
+    if (p.gstate == 0) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        const T fy0 = AXIS == 1 ? fe[r][0] : fo0, fz0 = AXIS == 1 ? fo0 : fe[r][0];
+        v[r] = cscale(v[r], fx0 * fy0 * fz0 * invN);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        const T fy0 = AXIS == 1 ? fe[r][0] : fo0, fy1 = AXIS == 1 ? fe[r][1] : fo1;
+        const T fz0 = AXIS == 1 ? fo0 : fe[r][0], fz1 = AXIS == 1 ? fo1 : fe[r][1];
+        const T ar = fx0 * fy0 - fx1 * fy1, ai = fx0 * fy1 + fx1 * fy0;
+        const T br = (ar * fz0 - ai * fz1) * invN, bi = (ar * fz1 + ai * fz0) * invN;
+        v[r] = cmul(mk(br, bi), v[r]);
+      }
+    }
+  }
+};
+
 template <typename T, int E, int AXIS, typename F>
 __device__ __forceinline__ void kstep_col(const Phys& p, cx<T>* v, int xi, int o, F epos, double invN_) {
   if (p.k_kind != K_SEP) {
@@ -459,35 +518,9 @@ __device__ __forceinline__ void kstep_col(const Phys& p, cx<T>* v, int xi, int o
                     invN_);
     return;
   }
-  const T invN = (T)invN_;
-  const auto* fxp = KTab<T>::get(p, 0) + 2 * xi;
-  const T fx0 = (T)fxp[0], fx1 = (T)fxp[1];
-  const auto* fop = KTab<T>::get(p, AXIS == 1 ? 2 : 1) + 2 * o;
-  const T fo0 = (T)fop[0], fo1 = (T)fop[1];
-  const auto* fvp = KTab<T>::get(p, AXIS);
-  T fe[E][2];
-#pragma unroll
-  for (int r = 0; r < E; ++r) {
-    const int e = epos(r);
-    fe[r][0] = (T)fvp[2 * e];
-    fe[r][1] = (T)fvp[2 * e + 1];
-  }
-  if (p.gstate == 0) {
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-      const T fy0 = AXIS == 1 ? fe[r][0] : fo0, fz0 = AXIS == 1 ? fo0 : fe[r][0];
-      v[r] = cscale(v[r], fx0 * fy0 * fz0 * invN);
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-      const T fy0 = AXIS == 1 ? fe[r][0] : fo0, fy1 = AXIS == 1 ? fe[r][1] : fo1;
-      const T fz0 = AXIS == 1 ? fo0 : fe[r][0], fz1 = AXIS == 1 ? fo1 : fe[r][1];
-      const T ar = fx0 * fy0 - fx1 * fy1, ai = fx0 * fy1 + fx1 * fy0;
-      const T br = (ar * fz0 - ai * fz1) * invN, bi = (ar * fz1 + ai * fz0) * invN;
-      v[r] = cmul(mk(br, bi), v[r]);
-    }
-  }
+  KCol<T, E, AXIS> kc;
+  kc.load(p, xi, o, epos);
+  kc.apply(p, v, invN_);
 }
 
 // ---------------------------------------------------------------------------
