@@ -85,7 +85,7 @@ __global__ void k_norm_sum(const cx<T>* psi, long N, double* partials) {
 // out[a] = tot[a] * scale; mode 1: out[0] = sqrt(tot[0] * scale) (the parSum
 // norm, src/kernels.cu:482).
 __device__ __forceinline__ double fin_sum(const double* __restrict__ p, int nb) {
-  return fin_sum_at(p, nb, threadIdx.x, blockDim.x);
+  return blockDim.x >= 1024 ? fin_sum8_at(p, nb, threadIdx.x, blockDim.x) : fin_sum_at(p, nb, threadIdx.x, blockDim.x);
 }
 
 __global__ void __launch_bounds__(1024) k_finalize(const double* partials, int nb, int na, int mode, double scale,
@@ -318,7 +318,9 @@ __global__ void __launch_bounds__(1024) k_finalize_b(const FinArgs* __restrict__
   __shared__ double sh[64];
   const int B = fin_block(f.nb);
   for (int a = 0; a < f.na; ++a) {
-    double acc[1] = {(int)threadIdx.x < B ? fin_sum_at(f.partials + (size_t)a * f.nb, f.nb, threadIdx.x, B) : 0.0};
+    const double* pa = f.partials + (size_t)a * f.nb;
+    double acc[1] = {(int)threadIdx.x >= B ? 0.0
+                     : B >= 1024 ? fin_sum8_at(pa, f.nb, threadIdx.x, B) : fin_sum_at(pa, f.nb, threadIdx.x, B)};
     block_sum<1>(acc, sh);
     if (threadIdx.x == 0) f.out[a] = f.mode == 1 ? sqrt(acc[0] * f.scale) : acc[0] * f.scale;
     __syncthreads();

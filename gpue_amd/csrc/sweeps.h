@@ -572,6 +572,22 @@ __device__ __forceinline__ double fin_sum_at(const double* __restrict__ p, int n
   return (s0 + s1) + (s2 + s3);
 }
 
+// The 1024-lane form (nb >= 4096): eight accumulators, so eight loads are in
+// flight per lane (a 32768-partial sum takes four rounds instead of eight).
+__device__ __forceinline__ double fin_sum8_at(const double* __restrict__ p, int nb, int t, int B) {
+  double s[8] = {};
+  int i = t;
+  for (; i + 7 * B < nb; i += 8 * B) {
+    double x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = p[i + k * B];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] += x[k];
+  }
+  for (; i < nb; i += B) s[0] += p[i];
+  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
 // The finalize block size for nb partials (launch_finalize).
 __host__ __device__ constexpr int fin_block(int nb) { return nb >= 4096 ? 1024 : 256; }
 
