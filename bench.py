@@ -230,13 +230,14 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
     done_at = [None] * n_inst
     while any(active) and max(steps) < max_steps:
         B.evolve(100, 0)
+        energies, _ = B.energy()   # every instance at once (one wait)
         for k, c in enumerate(ctxs):
             if not active[k]:
                 continue
             steps[k] += 100
             point_steps += 100 * c.N
             bytes_total += 100 * per_step_bytes[k]
-            e, _ = c.energy()
+            e = float(energies[k])
             if e_prev[k] is not None:
                 last_change[k] = abs(e - e_prev[k]) / abs(e_prev[k])
             if e_prev[k] is not None and abs(e - e_prev[k]) < tol * abs(e_prev[k]):

@@ -35,7 +35,8 @@ ABI_SYMBOLS = [
     "gpue_set_field_eqn", "gpue_upload_wfc", "gpue_download_wfc", "gpue_evolve", "gpue_parsum",
     "gpue_energy", "gpue_vortex_marker_2d", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
     "gpue_sync", "gpue_set_ramp", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
-    "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_evolve", "gpue_batch_last_error", "gpue_batch_destroy",
+    "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_evolve", "gpue_batch_energy", "gpue_batch_last_error",
+    "gpue_batch_destroy",
     "gpue_server_create", "gpue_server_submit", "gpue_server_poll", "gpue_server_drain", "gpue_server_clock",
     "gpue_server_destroy",
 ]
@@ -120,6 +121,7 @@ def lib():
         L.gpue_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
         L.gpue_batch_set_active.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.gpue_batch_evolve.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.gpue_batch_energy.argtypes = [vp, dp, dp]
         L.gpue_batch_last_error.argtypes = [vp]
         L.gpue_batch_last_error.restype = cp
         L.gpue_batch_destroy.argtypes = [vp]
@@ -363,7 +365,8 @@ class Context:
 
 class Batch:
     """Many independent contexts advanced together (gpue_batch): one launch per
-    sweep covers every instance of a shape; two steps are replayed from a graph."""
+    sweep covers every instance of a shape; each shape group replays two steps
+    from a graph on its own stream."""
 
     def __init__(self, contexts):
         self.contexts = list(contexts)
@@ -384,6 +387,13 @@ class Batch:
 
     def evolve(self, nsteps, gstate):
         self._check(lib().gpue_batch_evolve(self._h, nsteps, gstate), "batch_evolve")
+
+    def energy(self):
+        """(energy, mu) arrays of every instance, evaluated concurrently."""
+        e = np.zeros(len(self.contexts))
+        m = np.zeros(len(self.contexts))
+        self._check(lib().gpue_batch_energy(self._h, _dp(e), _dp(m)), "batch_energy")
+        return e, m
 
     def close(self):
         if getattr(self, "_h", None):

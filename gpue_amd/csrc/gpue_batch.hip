@@ -17,6 +17,7 @@
 // gpue_batch_create.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -70,6 +71,7 @@ struct gpue_batch {
   std::vector<int> cls;            // parity class: the instance's step parity at phase 0
   std::vector<Group> groups;
   int phase = 0;
+  double* terms = nullptr;         // pinned: 6 energy terms per instance (gpue_batch_energy)
   std::string err;
 };
 
@@ -216,6 +218,7 @@ void gpue_batch_destroy(gpue_batch* b) {
         }
   for (Group& g : b->groups)
     if (g.stream) (void)hipStreamDestroy(g.stream);
+  if (b->terms) (void)hipHostFree(b->terms);
   delete b;
 }
 
@@ -236,7 +239,8 @@ gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
     b->cls.push_back(c->next_step % 2);
   }
   b->device = ctxs[0]->p.device;
-  if (hipSetDevice(b->device) != hipSuccess) {
+  if (hipSetDevice(b->device) != hipSuccess ||
+      hipHostMalloc((void**)&b->terms, (size_t)n * 6 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     gpue_batch_destroy(b);
     return GPUE_EHIP;
   }
@@ -384,6 +388,23 @@ gpue_status gpue_batch_evolve(gpue_batch* b, int nsteps, int gstate) {
     if (gstate == 0) b->ctx[i]->pending_norm = true;
   }
   return GPUE_OK;
+}
+
+gpue_status gpue_batch_energy(gpue_batch* b, double* energy, double* mu) {
+  if (!b || !energy || !mu) return GPUE_EINVAL;
+  const size_t n = b->ctx.size();
+  for (size_t i = 0; i < n; ++i) {   // queue every instance's evaluation first
+    gpue_status st = energy_launch(b->ctx[i], b->terms + 6 * i);
+    if (st != GPUE_OK) return bfail(b, st, b->ctx[i]->err);
+  }
+  gpue_status out = GPUE_OK;
+  for (size_t i = 0; i < n; ++i) {
+    gpue_ctx* c = b->ctx[i];
+    BHIP(b, hipStreamSynchronize(c->stream));
+    energy_combine(c, b->terms + 6 * i, &energy[i], &mu[i]);
+    if (!std::isfinite(energy[i]) && out == GPUE_OK) out = bfail(b, GPUE_ENONFINITE, "energy is not finite");
+  }
+  return out;
 }
 
 }  // extern "C"

@@ -199,6 +199,10 @@ gpue_status gpue_batch_set_active(gpue_batch *b, int index, int active);
 /* Advance every active instance by nsteps steps (gstate 0 imaginary, 1 real);
  * each instance continues from its own step index (GPUE's gauge-order parity). */
 gpue_status gpue_batch_evolve(gpue_batch *b, int nsteps, int gstate);
+/* Energy and chemical potential of every instance (arrays of n, in the order
+ * of gpue_batch_create), evaluated on all instances' streams at once and
+ * waited for once: the values are those gpue_energy returns per context. */
+gpue_status gpue_batch_energy(gpue_batch *b, double *energy, double *mu);
 const char *gpue_batch_last_error(const gpue_batch *b);
 void        gpue_batch_destroy(gpue_batch *b);
 

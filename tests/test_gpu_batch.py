@@ -55,3 +55,16 @@ def test_batch_equals_alone(prec):
         assert np.array_equal(c.download(), a.download()), k
         assert c.energy() == a.energy()
     b.close()
+
+
+def test_batch_energy_equals_per_context():
+    ctxs = []
+    for k, (n, g, om) in enumerate([(64, 100.0, 0.3), (128, 300.0, 0.7), (64, 150.0, 0.0)]):
+        c, _, _ = make(n, g, om, 64 if k != 1 else 32, seed=k + 3)
+        ctxs.append(c)
+    b = G.Batch(ctxs)
+    b.evolve(12, 0)
+    e, m = b.energy()
+    for k, c in enumerate(ctxs):
+        assert (e[k], m[k]) == c.energy(), k
+    b.close()
