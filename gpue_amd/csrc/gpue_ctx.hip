@@ -830,21 +830,23 @@ static gpue_status time_graph(gpue_ctx* c, const Sweep* sw, const Phys& ph, int 
 
 // Time one sweep as the step runs it, in place; returns microseconds per
 // launch, including the dispatch gap between dependent launches.
-//  * States that fit the L2s (<= 32 MiB), and real time: each of `reps`
-//    launches is preceded by a copy of psi into the scratch buffer, and the
-//    same graph without the sweeps is timed and subtracted.  A launch on its
-//    own output would find its lines in the L2 of the XCD that wrote them,
-//    which no step does, and real-time phases of an unnormalised transform
-//    applied over and over grow into the slow range of sincos.
-//  * Larger states in imaginary time (no data-dependent path): `reps`
-//    launches in place on one copy.  Here the subtraction over-counts (a
-//    copy after a sweep runs slower than after a copy): measured against
-//    rocprofv3 kernel durations at 256^3, in place is within 2-5 %, the
-//    subtraction 7-25 % above.
+//  * States smaller than the 256 MiB last-level cache (MALL), and real time:
+//    each of `reps` launches is preceded by a copy of psi into the scratch
+//    buffer, and the same graph without the sweeps is timed and subtracted.
+//    A launch on its own output would find its lines in the caches it just
+//    wrote (the L2 of its XCD, the MALL), which no step does (at 512^2 this
+//    read 1.5 us low per sweep, for the 128 MiB FP32 256^3 state 18 % low),
+//    and real-time phases of an unnormalised transform applied over and over
+//    grow into the slow range of sincos.
+//  * States of 256 MiB and more in imaginary time (no data-dependent path):
+//    `reps` launches in place on one copy.  Here the subtraction over-counts
+//    (a copy after a sweep runs slower than after a copy): measured against
+//    rocprofv3 kernel durations of the FP64 256^3 job, in place is within
+//    2-5 %, the subtraction 7-25 % above.
 static gpue_status time_sweep(gpue_ctx* c, Sweep sw, const Phys& ph, int reps, hipEvent_t e0, hipEvent_t e1,
                               double* us) {
   sw.flags &= ~F_NORM;   // no reduction slots are written
-  if (c->N * c->esz > ((size_t)32 << 20) && ph.gstate == 0) {
+  if (c->N * c->esz >= ((size_t)256 << 20) && ph.gstate == 0) {
     HIPCHK(c, hipMemcpyAsync(c->tmp, c->psi, c->N * c->esz, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     gpue_status st = GPUE_OK;

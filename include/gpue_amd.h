@@ -157,8 +157,8 @@ gpue_status gpue_step_bytes(gpue_ctx *c, int gstate, double *bytes_per_step, dou
 /* Per-kernel cost of the step (no reference counterpart): each distinct sweep
  * of the gstate schedule is launched `reps` times back to back from a graph,
  * in place in the scratch buffer (psi is untouched): each launch on a fresh
- * copy of psi, the copies timed alone and subtracted, except for states over
- * 32 MiB in imaginary time, which run on one copy; HIP events on the context
+ * copy of psi, the copies timed alone and subtracted, except for states of
+ * 256 MiB and more in imaginary time, which run on one copy; HIP events on the context
  * stream.  Fills up to max_kinds entries: kernel name (owned by the
  * ctx, valid until the next call), microseconds per launch, launches per step
  * (averaged over the two gauge orders); *n_kinds = number of distinct sweeps. */
