@@ -85,11 +85,11 @@ def run_schedule(ctx, imag, real):
 # (each scaled by its reading for a plain 256 MiB device copy: 2.0 and 1.0), per
 # kernel instance and workload; the passes and tables are in profiles/.
 MEASURED_TRAFFIC = {
-    "gpue::col_sweep<double, 512, 5, 2>@512sq": 8803333,
-    "gpue::col_sweep<double, 512, 5, 3>@512sq": 8799003,
-    "gpue::row_sweep<double, 512, 3, 0>@512sq": 8676618,
-    "gpue::row_sweep<double, 512, 3, 1>@512sq": 8858157,
-    "gpue::row_sweep<double, 512, 3, 2>@512sq": 8810723,
+    "gpue::col_sweep<double, 512, 5, 2>@512sq": 8804762,
+    "gpue::col_sweep<double, 512, 5, 3>@512sq": 8789504,
+    "gpue::row_sweep<double, 512, 3, 0>@512sq": 8715878,
+    "gpue::row_sweep<double, 512, 3, 1>@512sq": 8894259,
+    "gpue::row_sweep<double, 512, 3, 2>@512sq": 8808243,
     "gpue::col_sweep<double, 256, 1, 0>@256cube": 536974439,
     "gpue::col_sweep<double, 256, 1, 1>@256cube": 536981941,
     "gpue::col_sweep<double, 256, 2, 2>@256cube": 537384725,
