@@ -70,7 +70,8 @@ struct gpue_ctx {
   std::string err;
   // vortex scratch
   unsigned char* vmask = nullptr;
-  double vmask_radius = -1.0;
+  double vmask_radius = 0.0;
+  bool vmask_valid = false;   // vmask holds the mask of vmask_radius
   int* vcand = nullptr;
   int* vmarker = nullptr;
   int* vfound = nullptr;

@@ -654,6 +654,7 @@ static std::map<std::string, double> name_map(const char* const* names, const do
 gpue_status gpue_set_field_eqn(gpue_ctx* c, int which, const char* expr, const char* const* names,
                                const double* values, int n_names) {
   if (!c || !expr || which < GPUE_FIELD_V || which > GPUE_FIELD_AZ) return GPUE_EINVAL;
+  if (n_names < 0 || (n_names > 0 && (!names || !values))) return GPUE_EINVAL;
   std::vector<EqnNodeC> nodes;
   bool td = false;
   std::string err;
@@ -1227,17 +1228,22 @@ gpue_status gpue_vortex_marker_2d(gpue_ctx* c, double radius, int* marker_out, i
   if (st != GPUE_OK) return st;
   const int xDim = c->dims[0];
   const size_t np = (size_t)(xDim - 1) * (xDim - 1);
-  if (!c->vmask) {
-    if (hipMalloc((void**)&c->vmask, np) != hipSuccess || hipMalloc((void**)&c->vcand, np * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&c->vmarker, (size_t)xDim * xDim * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&c->vfound, 64) != hipSuccess)
+  if (!c->vmask || !c->vcand || !c->vmarker || !c->vfound) {   // all four, or none after a failure
+    if ((!c->vmask && hipMalloc((void**)&c->vmask, np) != hipSuccess) ||
+        (!c->vcand && hipMalloc((void**)&c->vcand, np * sizeof(int)) != hipSuccess) ||
+        (!c->vmarker && hipMalloc((void**)&c->vmarker, (size_t)xDim * xDim * sizeof(int)) != hipSuccess) ||
+        (!c->vfound && hipMalloc((void**)&c->vfound, 64) != hipSuccess)) {
+      c->vmask_valid = false;
       return fail(c, GPUE_ENOMEM, "vortex scratch");
+    }
   }
-  if (radius != c->vmask_radius) {
+  if (!c->vmask_valid || radius != c->vmask_radius) {
     std::vector<unsigned char> m;
     vortex_mask(c->grid[0].data(), xDim, radius, m);
+    c->vmask_valid = false;
     HIPCHK(c, hipMemcpy(c->vmask, m.data(), np, hipMemcpyHostToDevice));
     c->vmask_radius = radius;
+    c->vmask_valid = true;
   }
   hipError_t e = c->prec == 64 ? launch_vortex<double>(c->psi, c->vmask, xDim, c->vcand, c->vmarker, c->vfound, c->stream)
                                : launch_vortex<float>(c->psi, c->vmask, xDim, c->vcand, c->vmarker, c->vfound, c->stream);
@@ -1372,7 +1378,7 @@ gpue_status gpue_math_eval(int device, int fn, const double* x, double* y, long 
 }
 
 gpue_status gpue_step_bytes(gpue_ctx* c, int gstate, double* bytes, double* sweeps) {
-  if (!c) return GPUE_EINVAL;
+  if (!c || (gstate != 0 && gstate != 1)) return GPUE_EINVAL;
   std::vector<Sweep> a, b;
   schedule(c, gstate, 0, false, a);
   schedule(c, gstate, 1, false, b);
@@ -1384,7 +1390,8 @@ gpue_status gpue_step_bytes(gpue_ctx* c, int gstate, double* bytes, double* swee
 
 gpue_status gpue_eqn_parse(const char* expr, const char* const* names, const double* values, int n_names,
                            gpue_eqn_node* nodes, int max_nodes, int* n_nodes, int* time_dependent) {
-  if (!expr) return GPUE_EINVAL;
+  if (!expr || (nodes && max_nodes < 0)) return GPUE_EINVAL;
+  if (n_names < 0 || (n_names > 0 && (!names || !values))) return GPUE_EINVAL;
   std::vector<EqnNodeC> out;
   bool td = false;
   std::string err;

@@ -59,3 +59,24 @@ def test_null_arguments():
     assert lib.gpue_create(None, None) == G.GPUE_EINVAL
     assert lib.gpue_energy(None, None, None) == G.GPUE_EINVAL
     assert lib.gpue_evolve(None, 0, 1, 0, 0, 0.0, None, 0, None) == G.GPUE_EINVAL
+
+
+def test_eqn_parse_rejects_bad_buffers():
+    # host-only entry point: checks its buffer arguments before writing any
+    lib = G.lib()
+    nodes = (G.EqnNode * 4)()
+    nn, td = ctypes.c_int(), ctypes.c_int()
+    assert lib.gpue_eqn_parse(b"x*x", None, None, 0, nodes, -1, ctypes.byref(nn), ctypes.byref(td)) == G.GPUE_EINVAL
+    names = (ctypes.c_char_p * 1)(b"a")
+    assert lib.gpue_eqn_parse(b"a*x", names, None, 1, None, 0, ctypes.byref(nn), ctypes.byref(td)) == G.GPUE_EINVAL
+    assert lib.gpue_eqn_parse(b"x*x", None, None, -1, None, 0, ctypes.byref(nn), ctypes.byref(td)) == G.GPUE_EINVAL
+    # a buffer too small gets what fits and GPUE_EINVAL; a large enough one the whole program
+    assert lib.gpue_eqn_parse(b"x*x+y", None, None, 0, None, 0, ctypes.byref(nn), ctypes.byref(td)) == G.GPUE_OK
+    need = nn.value
+    assert need > 1
+    small = (G.EqnNode * 1)()
+    assert lib.gpue_eqn_parse(b"x*x+y", None, None, 0, small, 1, ctypes.byref(nn), ctypes.byref(td)) == G.GPUE_EINVAL
+    big = (G.EqnNode * need)()
+    assert lib.gpue_eqn_parse(b"x*x+y", None, None, 0, big, need, ctypes.byref(nn), ctypes.byref(td)) == G.GPUE_OK
+    assert lib.gpue_eqn_eval(big, need, 2.0, 3.0, 0.0, 0.0) == 7.0
+    assert lib.gpue_step_bytes(None, 0, None, None) == G.GPUE_EINVAL
