@@ -33,7 +33,7 @@ A_ROTATION, A_CONSTANT = 0, 1
 ABI_SYMBOLS = [
     "gpue_params_default", "gpue_create", "gpue_destroy", "gpue_last_error", "gpue_set_field",
     "gpue_set_field_eqn", "gpue_upload_wfc", "gpue_download_wfc", "gpue_evolve", "gpue_parsum",
-    "gpue_energy", "gpue_vortex_marker_2d", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
+    "gpue_energy", "gpue_energy_consistent", "gpue_vortex_marker_2d", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
     "gpue_sync", "gpue_set_ramp", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
     "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_evolve", "gpue_batch_energy", "gpue_batch_last_error",
     "gpue_batch_destroy",
@@ -99,6 +99,7 @@ def lib():
                                   ctypes.c_int, ctypes.POINTER(EvolveResult)]
         L.gpue_parsum.argtypes = [vp]
         L.gpue_energy.argtypes = [vp, dp, dp]
+        L.gpue_energy_consistent.argtypes = [vp, dp, dp]
         L.gpue_vortex_marker_2d.argtypes = [vp, ctypes.c_double, ip, ip]
         L.gpue_vortex_list_2d.argtypes = [vp, ctypes.c_double, ip, ctypes.c_int, ip]
         L.gpue_get_grid.argtypes = [vp, ctypes.c_int, dp]
@@ -291,6 +292,14 @@ class Context:
         e = ctypes.c_double()
         m = ctypes.c_double()
         self._check(lib().gpue_energy(self._h, ctypes.byref(e), ctypes.byref(m)), "energy")
+        return e.value, m.value
+
+    def energy_consistent(self):
+        """(energy, mu) with the gauge term of the evolved Hamiltonian, hbar <A.k>
+        (spectral), in place of GPUE's forward-difference +i hbar A.grad."""
+        e = ctypes.c_double()
+        m = ctypes.c_double()
+        self._check(lib().gpue_energy_consistent(self._h, ctypes.byref(e), ctypes.byref(m)), "energy_consistent")
         return e.value, m.value
 
     def vortex_marker(self, radius):

@@ -300,7 +300,9 @@ gpue_status gpue::launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void
   if (ksum || ((s.flags & F_NORM) && !(s.flags & F_DEFER))) {
     const int nb = sweep_nb(c, s);
     const double dV = c->d[0] * c->d[1] * c->d[2];
-    e = ksum ? launch_finalize(c->partials, nb, 1, 0, s.invN, c->results + 1, c->stream)
+    // KSUM sums land in results[1] (kinetic) or, with F_ASUM, results[6 + axis]
+    e = ksum ? launch_finalize(c->partials, nb, 1, 0, s.invN, c->results + ((s.flags & F_ASUM) ? 6 + s.axis : 1),
+                               c->stream)
              : launch_finalize(c->partials, nb, 1, 1, dV, c->results + 0, c->stream);
     if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("finalize: ") + hipGetErrorString(e));
   }
@@ -788,6 +790,41 @@ gpue_status gpue_energy(gpue_ctx* c, double* energy, double* mu) {
   if (st != GPUE_OK) return st;
   HIPCHK(c, hipMemcpyAsync(r, c->results, sizeof(r), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  double E = 0, M = 0;
+  energy_combine(c, r, &E, &M);
+  if (energy) *energy = E;
+  if (mu) *mu = M;
+  if (!std::isfinite(E)) return fail(c, GPUE_ENONFINITE, "energy is not finite");
+  return GPUE_OK;
+}
+
+// GPUE's energy with its gauge term (+i hbar A.grad by forward differences,
+// src/split_op.cu:312-360) replaced by the term of the Hamiltonian the step
+// evolves: apply_gauge multiplies by exp(-A_a k_a Dt) between 1-D transforms
+// along axis a (src/operators.cu:1089-1110, src/evolution.cu:163-257), i.e.
+// hbar A_a k_a.  One Parseval sweep per gauge axis: FFT along a, then
+// sum A_a[gid] k_a |psi_hat|^2 / n_a over the mixed representation.
+gpue_status gpue_energy_consistent(gpue_ctx* c, double* energy, double* mu) {
+  if (!c) return GPUE_EINVAL;
+  gpue_status st = energy_launch(c, nullptr);
+  if (st != GPUE_OK) return st;
+  const gpue_params& p = c->p;
+  const bool gauge = p.corotating && p.dimnum > 1 && c->a_kind != A_ZERO;
+  // Az of the rotation gauge is 0; a table Az only when set and not all zero
+  const int naxes = !gauge ? 0 : (p.dimnum == 3 && c->a_kind == A_TABLE && !c->az_zero) ? 3 : 2;
+  const Phys ph = make_phys(c, 0, 0.0);
+  for (int a = 0; a < naxes && st == GPUE_OK; ++a)
+    st = launch(c, {a == 0, a, a == 0 ? (int)RM_KSUM : (int)CM_KSUM, F_ASUM, 1.0 / (double)c->dims[a]}, ph, c->psi,
+                c->psi);
+  if (st != GPUE_OK) return st;
+  double r[9];
+  HIPCHK(c, hipMemcpyAsync(r, c->results, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (p.corotating && p.dimnum > 1) {
+    double s = 0.0;
+    for (int a = 0; a < naxes; ++a) s += r[6 + a];
+    r[4] = p.hbar * s;
+  }
   double E = 0, M = 0;
   energy_combine(c, r, &E, &M);
   if (energy) *energy = E;

@@ -140,7 +140,9 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 #pragma unroll
         for (int r = 0; r < E; ++r) {
           const int xi = bpos<N, TPL, RFL>(tid, r);
-          acc += K_val(p, xi, yi, zi, gid_of(p, xi, yi, zi)) * abs2d(v[r]);
+          const long gid = gid_of(p, xi, yi, zi);
+          const double w = (flags & F_ASUM) ? A_val(p, 0, xi, yi, zi, gid) * p.pxg[xi] : K_val(p, xi, yi, zi, gid);
+          acc += w * abs2d(v[r]);
         }
       }
     }
@@ -243,7 +245,11 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
         for (int r = 0; r < E; ++r) {
           int yi, zi;
           yz(bpos<N, TPL, RFL>(tid, r), yi, zi);
-          acc += K_val(p, xi, yi, zi, gid_of(p, xi, yi, zi)) * abs2d(v[r]);
+          const long gid = gid_of(p, xi, yi, zi);
+          const double w = (flags & F_ASUM)
+                               ? A_val(p, a.axis, xi, yi, zi, gid) * (a.axis == 1 ? p.pyg[yi] : p.pzg[zi])
+                               : K_val(p, xi, yi, zi, gid);
+          acc += w * abs2d(v[r]);
         }
       }
     } else {

@@ -42,6 +42,8 @@ enum : unsigned {
                        // to norm_out, instead of a separate finalize
   F_DEFER = 64u,       // host only: the finalize of this F_NORM sweep is left to
                        // the next sweep (F_SCALE_PART)
+  F_ASUM = 128u,       // KSUM modes: weight A_axis * k_axis (the gauge term of the
+                       // evolved Hamiltonian) instead of K
 };
 
 // Time-dependent equation program (postfix): CONST pushes val, X/Y/Z/T push a

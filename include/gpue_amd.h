@@ -105,6 +105,16 @@ gpue_status gpue_parsum(gpue_ctx *c);
  * with the interaction weighted g instead of g/2). */
 gpue_status gpue_energy(gpue_ctx *c, double *energy, double *mu);
 
+/* energy_calc with its gauge term replaced by the one of the Hamiltonian the
+ * step evolves.  GPUE's energy adds +i hbar A.grad (forward differences,
+ * src/split_op.cu:312-360), while apply_gauge evolves exp(-A_a k_a Dt)
+ * (src/operators.cu:1089-1110), i.e. +hbar A.k: the two gauge terms have
+ * opposite signs.  Here hbar sum_a <A_a k_a> is taken spectrally (FFT along
+ * each gauge axis, Parseval), so E is the quantity real-time steps conserve and
+ * imaginary-time steps lower.  Equal to gpue_energy without a gauge term
+ * (corotating off, or 1D).  The -R ramp factor is not applied. */
+gpue_status gpue_energy_consistent(gpue_ctx *c, double *energy, double *mu);
+
 /* Tracker::findVortex (src/tracker.cu:102-161) on the current 2D psi:
  * marker_out (xDim*xDim ints, overwritten) and the count of hits. */
 gpue_status gpue_vortex_marker_2d(gpue_ctx *c, double mask_radius, int *marker_out, int *found);

@@ -631,6 +631,55 @@ double or_energy(or_ctx *c, double *mu_out) {
     return E * dg;
 }
 
+/* The gauge term of the Hamiltonian GPUE evolves.  apply_gauge multiplies by
+ * exp(-pA_a gdt) (imaginary) or exp(-i pA_a dt) (real) between 1-D transforms
+ * along axis a, pA_a = A_a k_a in the mixed representation (k_a along a,
+ * positions across; src/operators.cu:1089-1110, src/evolution.cu:163-257), so
+ * the evolved term is hbar A_a k_a.  energy_calc adds +i hbar A.grad instead
+ * (src/split_op.cu:312-360), i.e. minus that.  Here, by Parseval along each axis:
+ *   hbar * sum_a sum_mixed A_a[gid] k_a |FFT_a psi|^2 / n_a * dV. */
+double or_gauge_spectral(or_ctx *c) {
+    const or_params *p = &c->p;
+    const long N = c->N;
+    const int dims[3] = {p->xDim, p->yDim, p->zDim};
+    const double *A[3] = {c->Ax, c->Ay, c->Az};
+    const double *k[3] = {c->px, c->py, c->pz};
+    or_c *w = (or_c *)malloc(sizeof(or_c) * (size_t)N);
+    double total = 0.0;
+    for (int a = 0; a < p->dimnum; ++a) {
+        memcpy(w, c->wfc, sizeof(or_c) * (size_t)N);
+        fft_axis(w, p->xDim, p->yDim, p->zDim, a, -1);
+        double s = 0.0;
+        #pragma omp parallel for reduction(+ : s) schedule(static)
+        for (long gid = 0; gid < N; ++gid) {
+            const long idx = a == 0 ? gid % p->xDim : a == 1 ? (gid / p->xDim) % p->yDim
+                                                            : gid / ((long)p->xDim * p->yDim);
+            s += A[a][gid] * k[a][idx] * (w[gid].x * w[gid].x + w[gid].y * w[gid].y);
+        }
+        total += s / dims[a];
+    }
+    free(w);
+    return p->hbar * total * c->dx * c->dy * c->dz;
+}
+
+/* energy_calc with the gauge term replaced by or_gauge_spectral (when GPUE
+ * includes one: corotating and dimnum > 1) */
+double or_energy_consistent(or_ctx *c, double *mu_out) {
+    const int gauge = c->p.corotating && c->p.dimnum > 1;
+    const int corot = c->p.corotating;
+    c->p.corotating = 0;
+    double mu = 0.0;
+    double e = or_energy(c, &mu);
+    c->p.corotating = corot;
+    if (gauge) {
+        const double t = or_gauge_spectral(c);
+        e += t;
+        mu += t;
+    }
+    if (mu_out) *mu_out = mu;
+    return e;
+}
+
 /* src/evolution.cu:407-904 without host I/O: the step loop with energy cadence and
  * the imaginary-time early stop (:872-903).  Returns the number of steps taken. */
 int or_evolve(or_ctx *c, int nsteps, int gstate, int energy_every, double thr,

@@ -16,6 +16,8 @@
  *   or_energy          closed forms: HO ground state E = d/2 (src/unit_test.cu:1162-1278),
  *                      plane-wave kinetic energy, forward-difference gauge term on
  *                      analytic fields, Thomas-Fermi bound
+ *   or_gauge_spectral  plane waves (closed form hbar A k0 <1>), and the forward-difference
+ *                      term of or_energy on smooth fields (equal and opposite to O(dx))
  *   or_step/or_evolve  matlab/GPE_2d.m ordering transcribed independently in numpy
  *                      (tests/golden/make_golden.py), unitarity in real time,
  *                      linear HO ground state
@@ -70,6 +72,10 @@ double  or_dV(const or_ctx *c);
 void    or_step(or_ctx *c, int i, int gstate);
 void    or_parsum(or_ctx *c);
 double  or_energy(or_ctx *c, double *mu_out);
+/* the gauge term hbar <A.k> of the evolved Hamiltonian (spectral, per axis), and
+ * energy_calc with it in place of GPUE's +i hbar A.grad forward difference */
+double  or_gauge_spectral(or_ctx *c);
+double  or_energy_consistent(or_ctx *c, double *mu_out);
 int     or_evolve(or_ctx *c, int nsteps, int gstate, int energy_every, double thr,
                   double *energies, int max_energies, int *n_energies);
 

@@ -71,6 +71,10 @@ def lib():
         L.or_parsum.argtypes = [vp]
         L.or_energy.argtypes = [vp, dp]
         L.or_energy.restype = ctypes.c_double
+        L.or_gauge_spectral.argtypes = [vp]
+        L.or_gauge_spectral.restype = ctypes.c_double
+        L.or_energy_consistent.argtypes = [vp, dp]
+        L.or_energy_consistent.restype = ctypes.c_double
         L.or_evolve.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp,
                                 ctypes.c_int, ip]
         L.or_evolve.restype = ctypes.c_int
@@ -225,6 +229,16 @@ class Oracle:
     def energy(self):
         mu = ctypes.c_double()
         e = lib().or_energy(self._h, ctypes.byref(mu))
+        return e, mu.value
+
+    def gauge_spectral(self):
+        """hbar <A.k> of the evolved Hamiltonian (spectral, per axis)."""
+        return lib().or_gauge_spectral(self._h)
+
+    def energy_consistent(self):
+        """energy_calc with the gauge term replaced by gauge_spectral()."""
+        mu = ctypes.c_double()
+        e = lib().or_energy_consistent(self._h, ctypes.byref(mu))
         return e, mu.value
 
     def evolve(self, nsteps, gstate, energy_every=0, thr=-1.0, max_energies=100000):

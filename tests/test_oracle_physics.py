@@ -216,3 +216,61 @@ def test_ramp_imaginary_time_is_exponent_scaling():
     a.step(1, 0)
     b.step(1, 0)
     assert np.linalg.norm(a.wfc() - b.wfc()) <= 1e-13 * np.linalg.norm(b.wfc())
+
+
+def _vortex_gaussian(o, kappa=0.3):
+    X, Y = np.meshgrid(o.grid(0), o.grid(1))
+    return np.exp(-(X ** 2 + Y ** 2) / 2) * (X + 1j * Y) * np.exp(1j * kappa * X)
+
+
+@pytest.mark.parametrize("hbar,om,ox,oy", [(1.0, 0.7, 1.0, 1.0), (2.5, 0.9, 1.3, 0.6)])
+def test_gauge_spectral_angular_momentum_closed_form(hbar, om, ox, oy):
+    # Rotation gauge Ax = -y om ox, Ay = x om oy (src/operators.cu:630-647); the evolved
+    # term hbar <A.k> = hbar om (ox <-y kx> + oy <x ky>).  For psi = (x + iy) exp(-r^2/2)
+    # exp(i kappa x) (unnormalised), <x ky> = <-y kx> = pi/2 (the boost adds <-y kappa> = 0),
+    # so the term is hbar om (ox + oy) pi / 2.  The spectral derivative and the Riemann sum of
+    # a Gaussian are exact to rounding on this box.
+    o = O.Oracle(dimnum=2, xDim=128, yDim=128, xMax=8.0, yMax=8.0, corotating=1, omega=om,
+                 omegaX=ox, omegaY=oy, hbar=hbar)
+    o.set_wfc(_vortex_gaussian(o))
+    want = hbar * om * (ox + oy) * np.pi / 2
+    assert abs(o.gauge_spectral() - want) <= 1e-12 * want
+
+
+def test_gauge_spectral_against_forward_difference():
+    # GPUE's energy_calc gauge term (+i hbar A.grad, forward differences) is minus the
+    # evolved term up to the difference error: the sum of the two falls as dx^2 (the
+    # first-order error cancels on this field), and the consistent energy is GPUE's with
+    # the sign of its gauge term flipped, to that error.
+    gaps = []
+    for n in (64, 128):
+        o = O.Oracle(dimnum=2, xDim=n, yDim=n, xMax=8.0, yMax=8.0, corotating=1, omega=0.7)
+        o0 = O.Oracle(dimnum=2, xDim=n, yDim=n, xMax=8.0, yMax=8.0, corotating=0, omega=0.7)
+        psi = _vortex_gaussian(o)
+        o.set_wfc(psi)
+        o0.set_wfc(psi)
+        e_gpue, mu_gpue = o.energy()
+        e_none, mu_none = o0.energy()
+        e_cons, mu_cons = o.energy_consistent()
+        spec = o.gauge_spectral()
+        assert abs(e_cons - (e_none + spec)) <= 1e-12 * abs(e_cons)
+        assert abs(mu_cons - (mu_none + spec)) <= 1e-12 * abs(mu_cons)
+        gaps.append(abs((e_gpue - e_none) + spec) / spec)
+    assert gaps[0] < 0.03 and 3.5 < gaps[0] / gaps[1] < 4.5, gaps
+
+
+def test_energy_consistent_is_conserved_in_real_time():
+    # In an anisotropic rotating trap L_z is not conserved, so GPUE's energy (gauge term of
+    # the wrong sign) drifts while the energy of the evolved Hamiltonian stays put.
+    o = O.Oracle(dimnum=2, xDim=64, yDim=64, xMax=8.0, yMax=8.0, corotating=1, omega=0.5,
+                 omegaX=1.0, omegaY=1.6, dt=1e-3, gpe=0)
+    X, Y = np.meshgrid(o.grid(0), o.grid(1))
+    o.set_wfc(np.exp(-((X - 0.5) ** 2 + Y ** 2) / 2) * np.exp(0.4j * Y))
+    o.parsum()
+    e0, _ = o.energy_consistent()
+    g0, _ = o.energy()
+    o.evolve(400, 1)
+    e1, _ = o.energy_consistent()
+    g1, _ = o.energy()
+    assert abs(e1 - e0) < 1e-4 * abs(e0)
+    assert abs(g1 - g0) > 100 * abs(e1 - e0)
