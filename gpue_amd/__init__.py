@@ -24,8 +24,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgpue_amd.so")
 
-GPUE_OK, GPUE_EINVAL, GPUE_ENOMEM, GPUE_EUNSUPPORTED, GPUE_EHIP, GPUE_ENONFINITE, GPUE_EBUDGET, GPUE_EPARSE = range(8)
-STATUS_NAMES = ["OK", "EINVAL", "ENOMEM", "EUNSUPPORTED", "EHIP", "ENONFINITE", "EBUDGET", "EPARSE"]
+GPUE_OK, GPUE_EINVAL, GPUE_ENOMEM, GPUE_EUNSUPPORTED, GPUE_EHIP, GPUE_ENONFINITE, GPUE_EBUDGET, GPUE_EPARSE, GPUE_ETIMEOUT = range(9)
+STATUS_NAMES = ["OK", "EINVAL", "ENOMEM", "EUNSUPPORTED", "EHIP", "ENONFINITE", "EBUDGET", "EPARSE", "ETIMEOUT"]
 FIELD_V, FIELD_K, FIELD_AX, FIELD_AY, FIELD_AZ = range(5)
 A_ROTATION, A_CONSTANT = 0, 1
 
@@ -477,7 +477,7 @@ class Server:
     def drain(self, timeout_s=600.0):
         st = lib().gpue_server_drain(self._h, float(timeout_s))
         if st != GPUE_OK:
-            raise GpueError(st, "gpue_server_drain timed out")
+            raise GpueError(st, "gpue_server_drain timed out" if st == GPUE_ETIMEOUT else "gpue_server_drain")
 
     def clock(self):
         return float(lib().gpue_server_clock(self._h))

@@ -354,7 +354,7 @@ gpue_status gpue_server_drain(gpue_server* sv, double timeout_s) {
   if (!sv) return GPUE_EINVAL;
   const double t_end = sv->clock() + timeout_s;
   while (sv->inflight.load() > 0) {
-    if (sv->clock() > t_end) return GPUE_EBUDGET;
+    if (sv->clock() > t_end) return GPUE_ETIMEOUT;
     std::this_thread::sleep_for(std::chrono::microseconds(200));
   }
   return GPUE_OK;

@@ -33,7 +33,8 @@ typedef enum {
     GPUE_EHIP = 4,          /* HIP runtime error (message in gpue_last_error) */
     GPUE_ENONFINITE = 5,    /* norm or energy became NaN/Inf */
     GPUE_EBUDGET = 6,       /* serving: request exceeds the HBM budget */
-    GPUE_EPARSE = 7         /* equation string rejected */
+    GPUE_EPARSE = 7,        /* equation string rejected */
+    GPUE_ETIMEOUT = 8       /* serving: gpue_server_drain ran out of time */
 } gpue_status;
 
 enum { GPUE_FIELD_V = 0, GPUE_FIELD_K = 1, GPUE_FIELD_AX = 2, GPUE_FIELD_AY = 3, GPUE_FIELD_AZ = 4 };
@@ -251,7 +252,7 @@ gpue_status gpue_server_create(int device, double budget_bytes, int max_slots, g
 gpue_status gpue_server_submit(gpue_server *s, const gpue_request *r, unsigned long long id);
 /* Move up to max finished results to out; returns how many. Thread-safe. */
 int         gpue_server_poll(gpue_server *s, gpue_result *out, int max);
-/* Wait until every submitted request has finished (GPUE_EBUDGET on timeout). */
+/* Wait until every submitted request has finished (GPUE_ETIMEOUT on timeout). */
 gpue_status gpue_server_drain(gpue_server *s, double timeout_s);
 double      gpue_server_clock(const gpue_server *s);
 /* Finishes the requests in flight and queued, then stops the worker. */

@@ -115,3 +115,17 @@ def test_server_rejects_oversized():
     with pytest.raises(G.GpueError):
         sv.submit(1, _p(256))
     sv.close()
+
+
+def test_server_drain_timeout_status():
+    """drain reports GPUE_ETIMEOUT while requests are still running; they finish
+    normally afterwards."""
+    sv = G.Server(budget_bytes=4e9, max_slots=2)
+    sv.submit(1, _p(256, g=300.0), seed=1, max_imag_steps=20000, energy_tol=0.0, check_every=1000)
+    with pytest.raises(G.GpueError) as ei:
+        sv.drain(0.0)
+    assert ei.value.status == G.GPUE_ETIMEOUT
+    sv.drain(120)
+    res = sv.poll()
+    sv.close()
+    assert [r["status"] for r in res] == [0] and res[0]["imag_steps"] == 20000
