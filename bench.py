@@ -366,6 +366,7 @@ def main():
         el = t_end - t0
         sps = K / el
         e_final, mu_final = ctx.energy()
+        e_cons, _ = ctx.energy_consistent()   # gauge term of the evolved Hamiltonian (+hbar <A.k>)
         bps_imag, P_imag = ctx.step_bytes(0)
         bps_real, P_real = ctx.step_bytes(1)
         bytes_job = imag * bps_imag + real * bps_real
@@ -384,6 +385,7 @@ def main():
             "us_per_step_real": round((t_end - t_mid) / max(real, 1) * 1e6, 3),
             "effective_GBps": round(bytes_job / el / 1e9, 1), "sweeps_per_step_imag": P_imag,
             "sweeps_per_step_real": P_real, "energy_final": e_final, "mu_final": mu_final,
+            "energy_consistent_final": e_cons,
             "variants": list(variants), "kernels": rows}
     else:
         sps, el, roof = float("nan"), float("nan"), None
