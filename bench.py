@@ -371,6 +371,19 @@ def main():
         bps_real, P_real = ctx.step_bytes(1)
         bytes_job = imag * bps_imag + real * bps_real
         rows, bpl = kernel_table(ctx, 64, variants, {0: imag, 1: real}) if not args.no_profile else ([], 0)
+        # the boundary takes and returns host buffers: one upload of psi0 and one download
+        # of the result around the job give the PCIe-inclusive rate (never the headline)
+        ctx.sync()
+        t_a = time.perf_counter()
+        ctx.upload(psi0)
+        ctx.sync()
+        t_b = time.perf_counter()
+        ctx.download()
+        t_c = time.perf_counter()
+        pcie = {"upload_ms": round((t_b - t_a) * 1e3, 3), "download_ms": round((t_c - t_b) * 1e3, 3),
+                "steps_per_s_incl_transfers": round(K / (el + (t_c - t_a)), 1),
+                # GPUE's loop copies psi to the host at the start of every step (src/evolution.cu:430)
+                "steps_per_s_if_psi_downloaded_every_step": round(K / (el + K * (t_c - t_b)), 1)}
         ctx.close()
         log(f"512^2 FP64 gauge: {sps:.1f} steps/s ({el / K * 1e6:.2f} us/step; imaginary "
             f"{(t_mid - t0) / max(imag, 1) * 1e6:.2f}, real {(t_end - t_mid) / max(real, 1) * 1e6:.2f}), "
@@ -385,7 +398,7 @@ def main():
             "us_per_step_real": round((t_end - t_mid) / max(real, 1) * 1e6, 3),
             "effective_GBps": round(bytes_job / el / 1e9, 1), "sweeps_per_step_imag": P_imag,
             "sweeps_per_step_real": P_real, "energy_final": e_final, "mu_final": mu_final,
-            "energy_consistent_final": e_cons,
+            "energy_consistent_final": e_cons, "pcie_inclusive": pcie,
             "variants": list(variants), "kernels": rows}
     else:
         sps, el, roof = float("nan"), float("nan"), None
