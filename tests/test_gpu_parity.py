@@ -56,6 +56,11 @@ CASES = [
     ("2d-48", 2, dict(dimnum=2, xDim=48, yDim=48, xMax=6, yMax=6, g=50, gpe=1, corotating=1, omega=0.5)),
     ("3d-96", 3, dict(dimnum=3, xDim=96, yDim=48, zDim=48, xMax=8, yMax=6, zMax=6, g=60, gpe=1, omegaZ=1.5,
                       corotating=1, omega=0.4)),
+    # long strided lines along z: 1024 (radix 16,16,4 / 8,8,8,2) and 384 (12,8,4 / 12,4,4,2)
+    ("3d-longz", 3, dict(dimnum=3, xDim=32, yDim=16, zDim=1024, xMax=5, yMax=4, zMax=40, g=40, gpe=1, omegaZ=0.05,
+                         corotating=1, omega=0.3)),
+    ("3d-384z", 3, dict(dimnum=3, xDim=32, yDim=48, zDim=384, xMax=6, yMax=6, zMax=20, g=60, gpe=1, omegaZ=0.3,
+                        corotating=1, omega=0.4)),
 ]
 
 
@@ -246,8 +251,8 @@ def test_graph_replay_matches_eager(gstate, case):
 
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("name,dim,kw", [c for c in CASES if c[0] in ("1d-gauge", "2d-gauge", "3d-gauge", "2d-48",
-                                                                      "3d-96")],
-                         ids=["1d-gauge", "2d-gauge", "3d-gauge", "2d-48", "3d-96"])
+                                                                      "3d-96", "3d-longz")],
+                         ids=["1d-gauge", "2d-gauge", "3d-gauge", "2d-48", "3d-96", "3d-longz"])
 def test_every_kernel_variant(name, dim, kw, variant):
     # the autotuner may pick any plan and strided tile width per axis: each
     # must meet the FP64 bar, imaginary and real time
