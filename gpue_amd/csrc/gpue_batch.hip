@@ -164,7 +164,9 @@ static gpue_status launch_group_phase(gpue_batch* b, Group& g, int gs, int ph) {
         const int lx = sw.row ? c->dims[1] * c->dims[2] : c->dims[0];
         const int others = sw.row ? 1 : (sw.axis == 1 ? c->dims[2] : c->dims[1]);
         bool handled = false;
-        hipError_t e = launch_sweep_batched(c->prec, sw.row, n, c->var[sw.axis], sw.mode, s.dargs[k],
+        // FP32 real time: the kernels with FP64 phase factors, as the single-context path
+        const bool x = c->prec == 32 && gs != 0 && (sw.row || sw.mode == CM_K || sw.mode == CM_GAUGE);
+        hipError_t e = launch_sweep_batched(c->prec, sw.row, n, c->var[sw.axis], x ? (sw.mode | 16) : sw.mode, s.dargs[k],
                                             (int)g.members.size(), lx, others, g.stream, &handled);
         if (!handled || e != hipSuccess) return bfail(b, GPUE_EHIP, "batched sweep launch");
       }

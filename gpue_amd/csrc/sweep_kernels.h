@@ -58,8 +58,11 @@ __device__ __forceinline__ double abs2d(cx<T> v) {
 // ---------------------------------------------------------------------------
 // Row sweep: lines are contiguous x-rows, line = y + yDim*z.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int RV, int MODE>
+template <typename T, int N, int RV, int MODE_>
 __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) {
+  // bit 16 of the mode: FP32 real-time kernels forming phase factors in FP64
+  constexpr int MODE = MODE_ & 15;
+  constexpr bool X = (MODE_ & 16) != 0;
   constexpr int V = RV & 1;
   constexpr bool PRE = (RV >> 1) != 0;
   using P = Plan<N, V>;
@@ -92,10 +95,10 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
     }
     if constexpr (PRE) twr.load(tid, tw);
     fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
-    if (flags & F_HALFV_POST) halfV_all<T, E>(p, v, pos0);
+    if (flags & F_HALFV_POST) halfV_all<T, E, X>(p, v, pos0);
     if (flags & F_GAUGE_POST) {
       fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
-      gauge_all<T, E, 0>(p, v, posL, a.invN);
+      gauge_all<T, E, 0, X>(p, v, posL, a.invN);
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     }
   } else {
@@ -120,19 +123,19 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 #pragma unroll
         for (int r = 0; r < E; ++r) v[r] = scale_norm(v[r], inv);
       }
-      if (flags & F_HALFV_PRE) halfV_all<T, E>(p, v, pos0);
+      if (flags & F_HALFV_PRE) halfV_all<T, E, X>(p, v, pos0);
     }
     fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
     if constexpr (MODE == RM_GAUGE || MODE == RM_GAUGE_FWD) {
-      gauge_all<T, E, 0>(p, v, posL, a.invN);
+      gauge_all<T, E, 0, X>(p, v, posL, a.invN);
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     } else if constexpr (MODE == RM_FULL) {
-      kstep_all<T, E>(p, v, posL, a.invN);
+      kstep_all<T, E, X>(p, v, posL, a.invN);
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
-      if (flags & F_HALFV_POST) halfV_all<T, E>(p, v, pos0);
+      if (flags & F_HALFV_POST) halfV_all<T, E, X>(p, v, pos0);
       if (flags & F_GAUGE_POST) {
         fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
-        gauge_all<T, E, 0>(p, v, posL, a.invN);
+        gauge_all<T, E, 0, X>(p, v, posL, a.invN);
         fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
       }
     } else if constexpr (MODE == RM_KSUM) {
@@ -150,7 +153,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 
   // bridge modes continue with the next step's first row sweep
   if constexpr (MODE == RM_INV_FWD || MODE == RM_GAUGE_FWD) {
-    if (flags & F_HALFV_PRE) halfV_all<T, E>(p, v, pos0);
+    if (flags & F_HALFV_PRE) halfV_all<T, E, X>(p, v, pos0);
     fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
   }
 
@@ -179,8 +182,11 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 // Strided sweep: lines along y (axis 1, stride xDim) or z (axis 2, stride
 // xDim*yDim); a workgroup owns TC adjacent x-lines, lanes vary fastest in x.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int CV, int MODE>
+template <typename T, int N, int CV, int MODE_>
 __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) {
+  // bit 16 of the mode: FP32 real-time kernels forming phase factors in FP64
+  constexpr int MODE = MODE_ & 15;
+  constexpr bool X = (MODE_ & 16) != 0;
   constexpr int V = CV & 1;
   constexpr bool PRE = (CV >> 1) != 0;
   using P = Plan<N, V>;
@@ -254,11 +260,11 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
       }
     } else {
       if constexpr (MODE == CM_K) {
-        if (a.axis == 1) kstep_col<T, E, 1>(p, v, xq, o, epos, a.invN);
-        else kstep_col<T, E, 2>(p, v, xq, o, epos, a.invN);
+        if (a.axis == 1) kstep_col<T, E, 1, X>(p, v, xq, o, epos, a.invN);
+        else kstep_col<T, E, 2, X>(p, v, xq, o, epos, a.invN);
       } else {
-        if (a.axis == 1) gauge_all<T, E, 1>(p, v, posL, a.invN);
-        else gauge_all<T, E, 2>(p, v, posL, a.invN);
+        if (a.axis == 1) gauge_all<T, E, 1, X>(p, v, posL, a.invN);
+        else gauge_all<T, E, 2, X>(p, v, posL, a.invN);
       }
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
       if (active) {

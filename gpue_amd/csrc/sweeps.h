@@ -299,7 +299,13 @@ struct Pt {
 // once at load.  Only the reductions stay in FP64 in the FP32 form.
 
 // psi * exp(-arg) (imaginary time) or psi * exp(-i arg) (real time), times s
-template <typename T, int E>
+// X (FP32 real-time kernels only): the phase factor and the product in FP64,
+// rounded to FP32 once.  A factor rounded to FP32 carries a modulus error of
+// ~1e-7 that is the same at every step for static V, A and K, so the energy
+// error would grow linearly with the steps; the rounding of the product
+// depends on psi.  Separate kernels, so the imaginary-time ones keep their
+// registers.
+template <typename T, int E, bool X = false>
 __device__ __forceinline__ void apply_phase_all(cx<T>* v, const T (&arg)[E], int gstate, double s_) {
   const T s = (T)s_;
   if (gstate == 0) {
@@ -307,6 +313,31 @@ __device__ __forceinline__ void apply_phase_all(cx<T>* v, const T (&arg)[E], int
     for (int r = 0; r < E; ++r) {
       const T f = WMath<T>::ex(-arg[r]) * s;
       v[r] = cscale(v[r], f);
+    }
+  } else if constexpr (X && sizeof(T) == 4) {
+    double a[E];
+    bool small = true, fast = true;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      a[r] = -(double)arg[r];
+      small = small && sincos_small_ok(a[r]);
+      fast = fast && sincos_fast_ok(a[r]);
+    }
+    double sn[E], cs[E];
+    if (small) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) sincos_poly(a[r], &sn[r], &cs[r]);
+    } else if (fast) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) sincos_core(a[r], &sn[r], &cs[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < E; ++r) sincos_cw(a[r], &sn[r], &cs[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const double x = v[r].x, y = v[r].y, c = cs[r] * s_, d = sn[r] * s_;
+      v[r] = mk((float)(c * x - d * y), (float)(c * y + d * x));
     }
   } else {
     // one range check for the lane's E arguments, then branch-free cores; a
@@ -360,7 +391,7 @@ __device__ __forceinline__ void halfV_vargs(const Phys& p, F pos, T (&arg)[E]) {
   for (int r = 0; r < E; ++r) arg[r] = arg[r] * cV;
 }
 
-template <typename T, int E>
+template <typename T, int E, bool X = false>
 __device__ __forceinline__ void halfV_apply(const Phys& p, cx<T>* v, T (&arg)[E]) {
   if (p.gpe) {
     const T g = (T)p.g, cH = (T)p.cH;
@@ -370,14 +401,14 @@ __device__ __forceinline__ void halfV_apply(const Phys& p, cx<T>* v, T (&arg)[E]
       arg[r] += g * d * cH;
     }
   }
-  apply_phase_all<T, E>(v, arg, p.gstate, 1.0);
+  apply_phase_all<T, E, X>(v, arg, p.gstate, 1.0);
 }
 
-template <typename T, int E, typename F>
+template <typename T, int E, bool X = false, typename F>
 __device__ __forceinline__ void halfV_all(const Phys& p, cx<T>* v, F pos) {
   T arg[E];
   halfV_vargs<T, E>(p, pos, arg);
-  halfV_apply<T, E>(p, v, arg);
+  halfV_apply<T, E, X>(p, v, arg);
 }
 
 // gauge operator along AXIS: GpA*/EpA* (src/operators.cu:1089-1110) with both
@@ -412,11 +443,11 @@ __device__ __forceinline__ void gauge_args(const Phys& p, F pos, T (&arg)[E]) {
   }
 }
 
-template <typename T, int E, int AXIS, typename F>
+template <typename T, int E, int AXIS, bool X = false, typename F>
 __device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double invn) {
   T arg[E];
   gauge_args<T, E, AXIS>(p, pos, arg);
-  apply_phase_all<T, E>(v, arg, p.gstate, invn);
+  apply_phase_all<T, E, X>(v, arg, p.gstate, invn);
 }
 
 // Per-axis K factor table in the working precision.
@@ -432,9 +463,26 @@ struct KTab<float> {
 // K step: GK/EK (src/operators.cu:1082,1102) with both 1/sqrt(N) of
 // src/evolution.cu:761,777 folded into invN.  For the separable K the operator
 // is the product of per-axis factors (no transcendental per point).
-template <typename T, int E, typename F>
+template <typename T, int E, bool X = false, typename F>
 __device__ __forceinline__ void kstep_all(const Phys& p, cx<T>* v, F pos, double invN_) {
   const T invN = (T)invN_;
+  if constexpr (X && sizeof(T) == 4) {
+    if (p.k_kind == K_SEP && p.gstate != 0) {
+      // factor from the FP64 tables, product in FP64 (see apply_phase_all)
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        const Pt q = pos(r);
+        const double* gx = p.kf[0] + 2 * q.x;
+        const double* gy = p.kf[1] + 2 * q.y;
+        const double* gz = p.kf[2] + 2 * q.z;
+        const double ar = gx[0] * gy[0] - gx[1] * gy[1], ai = gx[0] * gy[1] + gx[1] * gy[0];
+        const double br = (ar * gz[0] - ai * gz[1]) * invN_, bi = (ar * gz[1] + ai * gz[0]) * invN_;
+        const double x = v[r].x, y = v[r].y;
+        v[r] = mk((T)(br * x - bi * y), (T)(br * y + bi * x));
+      }
+      return;
+    }
+  }
   if (p.k_kind == K_SEP) {
     T f[E][6];
 #pragma unroll
@@ -466,7 +514,7 @@ __device__ __forceinline__ void kstep_all(const Phys& p, cx<T>* v, F pos, double
     const Pt q = pos(r);
     arg[r] = (T)p.Kt[gid_of(p, q.x, q.y, q.z)] * cK;
   }
-  apply_phase_all<T, E>(v, arg, p.gstate, invN_);
+  apply_phase_all<T, E, X>(v, arg, p.gstate, invN_);
 }
 
 // K step on a strided line (axis AXIS = 1 or 2 varies along the line; x = xi
@@ -513,11 +561,11 @@ struct KCol {
   }
 };
 
-template <typename T, int E, int AXIS, typename F>
+template <typename T, int E, int AXIS, bool X = false, typename F>
 __device__ __forceinline__ void kstep_col(const Phys& p, cx<T>* v, int xi, int o, F epos, double invN_) {
-  if (p.k_kind != K_SEP) {
-    kstep_all<T, E>(p, v, [&](int r) { const int e = epos(r); return AXIS == 1 ? Pt{xi, e, o} : Pt{xi, o, e}; },
-                    invN_);
+  if (p.k_kind != K_SEP || (X && sizeof(T) == 4 && p.gstate != 0)) {
+    kstep_all<T, E, X>(p, v, [&](int r) { const int e = epos(r); return AXIS == 1 ? Pt{xi, e, o} : Pt{xi, o, e}; },
+                       invN_);
     return;
   }
   KCol<T, E, AXIS> kc;
