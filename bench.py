@@ -150,27 +150,81 @@ def roofline(rows, bytes_per_launch, workload):
             "avg_launch_us": top["us_per_launch"]}
 
 
-def cpu_baseline(budget_s):
+def physical_cores():
+    """Physical cores of the host per lscpu (distinct core, socket pairs), or None."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=20).stdout
+    except Exception:
+        return None
+    pairs = {line.strip() for line in out.splitlines() if line and not line.startswith("#")}
+    return len(pairs) or None
+
+
+def cpu_baseline(blocks=5, block_steps=50):
+    """The FP64 oracle (OpenMP) on the host: the headline job's imaginary-time
+    and real-time steps timed separately in blocks of block_steps steps (median
+    of `blocks` blocks each), with every thread this process may use and with
+    one; the job rate is 14000 / (4000 / r_imag + 10000 / r_real)."""
     from oracle import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    share = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(int(env), share) if env.isdigit() and int(env) > 0 else share
     orc = O.Oracle(**CFG2)
     grids = [orc.grid(0), orc.grid(1)]
     psi = seeded_gaussian((512, 512), grids, 1, True, 0.0)
     psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
-    orc.set_wfc(psi)
-    steps = 0
+
+    def rates(nt):
+        O.set_threads(nt)
+        orc.set_wfc(psi)
+        out = {}
+        step = 0
+        for gstate in (0, 1):
+            per = []
+            for _ in range(blocks):
+                t0 = time.perf_counter()
+                for i in range(block_steps):
+                    orc.step(step + i, gstate)
+                per.append(block_steps / (time.perf_counter() - t0))
+                step += block_steps
+            out[gstate] = float(np.median(per))
+        job = 14000.0 / (4000.0 / out[0] + 10000.0 / out[1])
+        return job, out
+
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s or steps < 20:
-        for i in range(10):
-            orc.step(steps + i, 0 if steps < 4000 else 1)
-        steps += 10
+    job_n, ph_n = rates(threads)
+    job_1, ph_1 = rates(1)
+    O.set_threads(threads)
     el = time.perf_counter() - t0
-    return {"value": round(steps / el, 3), "unit": "steps/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
-            "kind": "port",
-            "sample": f"FP64 oracle (oracle/gpue_oracle.c, OpenMP) on the same 512^2 gauge workload: first "
-                      f"{steps} imaginary-time steps of the job, {el:.1f} s"}
+    return {"value": round(job_n, 3), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (oracle/gpue_oracle.c, OpenMP, {threads} threads) on the 512^2 gauge job: "
+                      f"median of {blocks} blocks of {block_steps} imaginary-time steps ({ph_n[0]:.1f} steps/s) and "
+                      f"of {blocks} blocks of {block_steps} real-time steps ({ph_n[1]:.1f} steps/s), combined with the "
+                      f"job's 4000/10000 split; {el:.1f} s of CPU timing in total",
+            "one_core": {"value": round(job_1, 3), "imag_steps_per_s": round(ph_1[0], 3),
+                         "real_steps_per_s": round(ph_1[1], 3)},
+            "imag_steps_per_s": round(ph_n[0], 3), "real_steps_per_s": round(ph_n[1], 3),
+            "physical_cores_lscpu": physical_cores(), "cpus_allowed": share}
+
+
+def config4_specs(n_inst=64, seed=4):
+    """The batched sweep's instances: (sizes, [(problem, psi0 seed)]).  Grids
+    are a seeded mix of 128^2-512^2, g ~ U[100,2000], Omega ~ U[0,0.95] through
+    the gauge operators, omega_y/omega_x ~ U[0.8,1.2]."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([128, 192, 256, 384, 512], n_inst)
+    specs = []
+    for k in range(n_inst):
+        n = int(sizes[k])
+        L = 10.0 if n <= 256 else 12.0
+        cfg = dict(dimnum=2, xDim=n, yDim=n, xMax=L, yMax=L, g=float(rng.uniform(100, 2000)), gpe=1,
+                   omega=float(rng.uniform(0, 0.95)), corotating=1, omegaX=1.0,
+                   omegaY=float(rng.uniform(0.8, 1.2)), gdt=1e-3, dt=1e-4)
+        specs.append((cfg, 100 + k))
+    return sizes, specs
 
 
 def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
@@ -179,17 +233,12 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
     per shape, two steps per graph replay) until each instance's energy changes
     by less than tol (relative) over 100 steps (at most max_steps).  psi0 is
     the noisy Gaussian exp(-r^2/2)(1 + 0.01 U[-1,1]), seeded per instance."""
-    rng = np.random.default_rng(seed)
-    sizes = rng.choice([128, 192, 256, 384, 512], n_inst)
+    sizes, specs = config4_specs(n_inst, seed)
     ctxs = []
     tuned = {}
-    for k in range(n_inst):
+    for k, (cfg, psi_seed) in enumerate(specs):
         n = int(sizes[k])
-        L = 10.0 if n <= 256 else 12.0
-        cfg = dict(dimnum=2, xDim=n, yDim=n, xMax=L, yMax=L, g=float(rng.uniform(100, 2000)), gpe=1,
-                   omega=float(rng.uniform(0, 0.95)), corotating=1, omegaX=1.0,
-                   omegaY=float(rng.uniform(0.8, 1.2)), gdt=1e-3, dt=1e-4)
-        ctx, _ = make_ctx(G, 32, cfg, seed=100 + k, phase=False, noise=0.01)
+        ctx, _ = make_ctx(G, 32, cfg, seed=psi_seed, phase=False, noise=0.01)
         if n not in tuned:
             tuned[n] = ctx.autotune()
         else:
@@ -324,7 +373,6 @@ def main():
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the per-kernel graph replays and the roofline (for runs under a kernel tracer, "
                          "whose graph interception has crashed on the host)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--only", choices=["512", "256c", "lat", "batch", "serve", "copy"], default=None,
                     help="profiling: run one workload only (copy: a plain 256 MiB device copy, the "
                          "calibration kernel for the HBM byte counters)")
@@ -355,6 +403,9 @@ def main():
         variants = tune(ctx, args)
         run_schedule(ctx, W // 2, W - W // 2)
         ctx.upload(psi0)
+        # the step blocks the timed calls replay are captured here, not inside the timing
+        ctx.prepare_evolve(imag, 0)
+        ctx.prepare_evolve(real, 1)
         ctx.sync()
         t0 = time.perf_counter()
         ctx.evolve(imag, 0)
@@ -367,6 +418,15 @@ def main():
         sps = K / el
         e_final, mu_final = ctx.energy()
         e_cons, _ = ctx.energy_consistent()   # gauge term of the evolved Hamiltonian (+hbar <A.k>)
+        # the same job with every sweep launched on its own (no graphs), for comparison
+        ctx.set_graph_steps(0)
+        ctx.upload(psi0)
+        ctx.sync()
+        t_e0 = time.perf_counter()
+        run_schedule(ctx, imag, real)
+        ctx.sync()
+        eager_sps = K / (time.perf_counter() - t_e0)
+        ctx.set_graph_steps(args.graph_steps)
         bps_imag, P_imag = ctx.step_bytes(0)
         bps_real, P_real = ctx.step_bytes(1)
         bytes_job = imag * bps_imag + real * bps_real
@@ -399,6 +459,7 @@ def main():
             "effective_GBps": round(bytes_job / el / 1e9, 1), "sweeps_per_step_imag": P_imag,
             "sweeps_per_step_real": P_real, "energy_final": e_final, "mu_final": mu_final,
             "energy_consistent_final": e_cons, "pcie_inclusive": pcie,
+            "graph_steps": args.graph_steps, "eager_steps_per_s": round(eager_sps, 1),
             "variants": list(variants), "kernels": rows}
     else:
         sps, el, roof = float("nan"), float("nan"), None
@@ -480,8 +541,10 @@ def main():
 
     cpu = None
     if not args.no_cpu and args.only is None:
-        cpu = cpu_baseline(args.cpu_seconds)
-        log(f"CPU baseline: {cpu['value']} steps/s on {cpu['cores']} threads")
+        cpu = cpu_baseline()
+        log(f"CPU baseline: {cpu['value']} steps/s on {cpu['cores']} threads, {cpu['one_core']['value']} on one")
+        if sps == sps:
+            extra["speedup_vs_cpu_baseline"] = round(sps / cpu["value"], 1)
 
     out = {
         "metric": "split-op steps/s (512^2 rotating condensate, FP64, gauge Omega=0.9)",

@@ -34,7 +34,7 @@ ABI_SYMBOLS = [
     "gpue_params_default", "gpue_create", "gpue_destroy", "gpue_last_error", "gpue_set_field",
     "gpue_set_field_eqn", "gpue_upload_wfc", "gpue_download_wfc", "gpue_evolve", "gpue_parsum",
     "gpue_energy", "gpue_energy_consistent", "gpue_vortex_marker_2d", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
-    "gpue_sync", "gpue_set_ramp", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
+    "gpue_sync", "gpue_set_ramp", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_prepare_evolve", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
     "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_evolve", "gpue_batch_energy", "gpue_batch_last_error",
     "gpue_batch_destroy",
     "gpue_server_create", "gpue_server_submit", "gpue_server_poll", "gpue_server_drain", "gpue_server_clock",
@@ -109,6 +109,7 @@ def lib():
         L.gpue_autotune.argtypes = [vp, ctypes.c_int, ip]
         L.gpue_set_variants.argtypes = [vp, ip]
         L.gpue_set_graph_steps.argtypes = [vp, ctypes.c_int]
+        L.gpue_prepare_evolve.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.gpue_set_timing.argtypes = [vp, ctypes.c_int]
         L.gpue_timing_get.argtypes = [vp, ctypes.c_int, ctypes.POINTER(cp), ctypes.POINTER(ctypes.c_long), dp]
         L.gpue_step_bytes.argtypes = [vp, ctypes.c_int, dp, dp]
@@ -325,6 +326,10 @@ class Context:
 
     def set_graph_steps(self, steps):
         self._check(lib().gpue_set_graph_steps(self._h, steps), "set_graph_steps")
+
+    def prepare_evolve(self, nsteps, gstate, first_step=0, energy_every=0):
+        """Capture the step blocks evolve(nsteps, gstate, first_step, energy_every) replays."""
+        self._check(lib().gpue_prepare_evolve(self._h, first_step, nsteps, gstate, energy_every), "prepare_evolve")
 
     def sweep_profile(self, gstate, reps=50):
         """Per-kernel cost of the step: [{name, us, per_step}] from back-to-back

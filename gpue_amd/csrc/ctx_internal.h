@@ -23,6 +23,15 @@ struct Sweep {
 
 constexpr int TIMING_PAIRS = 256;
 
+// A captured block of `len` steps of one gstate (replayed from an even step).
+struct StepGraph {
+  int gstate = 0, len = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  long last_use = 0;
+};
+constexpr int MAX_STEP_GRAPHS = 8;
+
 }  // namespace gpue
 
 struct gpue_ctx {
@@ -40,11 +49,13 @@ struct gpue_ctx {
   std::map<int, void*> tw;   // 2*N + plan -> pass twiddle table (TwTab) in the working precision
   int var[3] = {0, 0, 0};      // kernel variant per axis (row sweeps: axis 0)
   bool tuned = false;
-  // steady-state step graphs: graph_steps consecutive steps (even count, starting
-  // at an even step, norm pending) captured once per gstate and replayed
+  // steady-state step graphs: blocks of up to graph_steps consecutive steps (even
+  // count, starting at an even step, norm pending) captured once per (gstate,
+  // length) and replayed; shorter blocks cover the tail of an evolve call and the
+  // steps between energy evaluations
   int graph_steps = 8;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  hipGraph_t graph[2] = {nullptr, nullptr};
+  std::vector<gpue::StepGraph> graphs;
+  long graph_clock = 0;        // use counter for the least-recently-used eviction
   double* partials = nullptr;
   size_t partials_n = 0;
   double* results = nullptr;   // [0] norm  [1] kinetic  [2..5] position terms

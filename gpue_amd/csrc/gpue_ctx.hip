@@ -321,14 +321,17 @@ gpue_status gpue::apply_pending(gpue_ctx* c) {
 
 static bool valid_dim(int n) { return sweep_supported(n); }
 
-// Any change of fields, parameters or kernel variants invalidates the captured graphs.
+// Any change of fields, parameters or kernel variants invalidates the captured
+// graphs.  Replays may still be queued (evolve returns without waiting in real
+// time), so the stream is drained before an executable graph is destroyed.
 static void drop_graphs(gpue_ctx* c) {
-  for (int g = 0; g < 2; ++g) {
-    if (c->gexec[g]) (void)hipGraphExecDestroy(c->gexec[g]);
-    if (c->graph[g]) (void)hipGraphDestroy(c->graph[g]);
-    c->gexec[g] = nullptr;
-    c->graph[g] = nullptr;
+  if (c->graphs.empty()) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (StepGraph& g : c->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
   }
+  c->graphs.clear();
 }
 
 // Per-axis terms of kharmonic_V (src/operators.cu:994-1008) with the rotation
@@ -435,10 +438,7 @@ void gpue_destroy(gpue_ctx* c) {
   for (auto& kv : c->tw) (void)hipFree(kv.second);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
-  for (int g = 0; g < 2; ++g) {
-    if (c->gexec[g]) (void)hipGraphExecDestroy(c->gexec[g]);
-    if (c->graph[g]) (void)hipGraphDestroy(c->graph[g]);
-  }
+  drop_graphs(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1097,13 +1097,13 @@ void gpue::schedule_block(const gpue_ctx* c, int gstate, int first, int n, std::
 
 extern "C" {
 
-// Capture graph_steps steady-state steps (starting on an even step, norm
-// pending in imaginary time) into a graph for this gstate.
-static gpue_status build_graph(gpue_ctx* c, int gstate) {
+// Capture `len` steady-state steps (starting on an even step, norm pending in
+// imaginary time) into a graph for this gstate; returns the cache entry.
+static gpue_status build_graph(gpue_ctx* c, int gstate, int len, StepGraph** out) {
   std::vector<Sweep> sched;
   std::vector<int> step_of;
   const double Dt = gstate == 0 ? c->p.gdt : c->p.dt;
-  schedule_block(c, gstate, 0, c->graph_steps, sched, step_of);
+  schedule_block(c, gstate, 0, len, sched, step_of);
   if (gstate == 0)
     for (size_t k = 0; k + 1 < sched.size(); ++k)
       if (step_of[k + 1] == step_of[k] + 1) defer_norm(c, sched[k], sched[k + 1]);
@@ -1126,9 +1126,49 @@ static gpue_status build_graph(gpue_ctx* c, int gstate) {
     (void)hipGraphDestroy(g);
     return fail(c, GPUE_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
   }
-  c->graph[gstate] = g;
-  c->gexec[gstate] = ge;
+  if ((int)c->graphs.size() >= MAX_STEP_GRAPHS) {   // evict the least recently used block
+    size_t lru = 0;
+    for (size_t k = 1; k < c->graphs.size(); ++k)
+      if (c->graphs[k].last_use < c->graphs[lru].last_use) lru = k;
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // its last replay may still be queued
+    (void)hipGraphExecDestroy(c->graphs[lru].exec);
+    (void)hipGraphDestroy(c->graphs[lru].graph);
+    c->graphs.erase(c->graphs.begin() + lru);
+  }
+  StepGraph sg;
+  sg.gstate = gstate;
+  sg.len = len;
+  sg.graph = g;
+  sg.exec = ge;
+  c->graphs.push_back(sg);
+  *out = &c->graphs.back();
   return GPUE_OK;
+}
+
+static gpue_status step_graph(gpue_ctx* c, int gstate, int len, StepGraph** out) {
+  for (StepGraph& g : c->graphs)
+    if (g.gstate == gstate && g.len == len) {
+      *out = &g;
+      return GPUE_OK;
+    }
+  return build_graph(c, gstate, len, out);
+}
+
+// Steps of a captured block starting at step s of an evolve call (0: the step
+// is launched on its own): up to G steps (an even count, from an even step) when
+// the fields do not depend on time, ending before the next energy evaluation and
+// at the end of the call.
+static int block_len(const gpue_ctx* c, int first_step, int nsteps, int s, int energy_every) {
+  const int G = c->graph_steps;
+  const int i = first_step + s;
+  if (!(G > 0 && !c->timing && !c->v_time && !c->ramp_type && (i % 2) == 0)) return 0;
+  int L = std::min(G, nsteps - s);
+  if (energy_every > 0) {
+    const int nxt = ((i + energy_every - 1) / energy_every) * energy_every;   // first cadence step >= i
+    L = std::min(L, nxt - i);
+  }
+  L &= ~1;
+  return L >= 2 ? L : 0;
 }
 
 static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gstate, int energy_every,
@@ -1145,32 +1185,22 @@ static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gsta
   std::vector<Sweep> sched;
   const double Dt = gstate == 0 ? c->p.gdt : c->p.dt;
   int s;
-  const int G = c->graph_steps;
   gpue_status st0 = settle_deferred(c);
   if (st0 != GPUE_OK) return st0;
-  // steady state: step s starts a replay of a captured block of G steps when
-  // no energy cadence falls inside it and the fields do not depend on time
-  // (the imaginary-time graph always rescales on entry: results[0] holds the
+  // steady state: captured blocks of steps (block_len) are replayed; the
+  // imaginary-time graph always rescales on entry (results[0] holds the
   // pending norm or 1.0, and x*1.0 is exact)
-  auto uses_graph = [&](int s2) {
-    const int i2 = first_step + s2;
-    if (!(G > 0 && !c->timing && !c->v_time && !c->ramp_type && (i2 % 2) == 0 && nsteps - s2 >= G)) return false;
-    if (energy_every > 0) {
-      const int nxt = ((i2 + energy_every - 1) / energy_every) * energy_every;
-      if (nxt < i2 + G) return false;
-    }
-    return true;
-  };
   for (s = 0; s < nsteps; ++s) {
     const int i = first_step + s;
-    if (uses_graph(s)) {
-      if (!c->gexec[gstate]) {
-        gpue_status st = build_graph(c, gstate);
-        if (st != GPUE_OK) return st;
-      }
-      HIPCHK(c, hipGraphLaunch(c->gexec[gstate], c->stream));
+    const int L = block_len(c, first_step, nsteps, s, energy_every);
+    if (L > 0) {
+      StepGraph* sg = nullptr;
+      gpue_status st = step_graph(c, gstate, L, &sg);
+      if (st != GPUE_OK) return st;
+      sg->last_use = ++c->graph_clock;
+      HIPCHK(c, hipGraphLaunch(sg->exec, c->stream));
       if (gstate == 0) c->pending_norm = true;
-      s += G - 1;
+      s += L - 1;
       continue;
     }
     Phys ph = make_phys(c, gstate, Dt * i);   // time = Dt*i (src/evolution.cu:408)
@@ -1191,7 +1221,7 @@ static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gsta
       }
     }
     // leave this step's norm to the next one when that is an eager step too
-    if (gstate == 0 && c->p.dimnum >= 2 && s + 1 < nsteps && !uses_graph(s + 1) &&
+    if (gstate == 0 && c->p.dimnum >= 2 && s + 1 < nsteps && block_len(c, first_step, nsteps, s + 1, energy_every) == 0 &&
         !(energy_every > 0 && i % energy_every == 0) && (sched.back().flags & F_NORM)) {
       const int nb = sweep_nb(c, sched.back());
       if (nb < 4096) {
@@ -1245,6 +1275,26 @@ gpue_status gpue_evolve(gpue_ctx* c, int first_step, int nsteps, int gstate, int
                         double* energies, int max_energies, gpue_evolve_result* res) {
   if (!c || nsteps < 0 || (gstate != 0 && gstate != 1) || first_step < 0) return GPUE_EINVAL;
   return evolve_impl(c, first_step, nsteps, gstate, energy_every, energy_thr, energies, max_energies, res, true);
+}
+
+
+// Capture, without launching anything, every step block an evolve call with
+// these arguments replays, so the call itself spends no time on captures.
+gpue_status gpue_prepare_evolve(gpue_ctx* c, int first_step, int nsteps, int gstate, int energy_every) {
+  if (!c || nsteps < 0 || (gstate != 0 && gstate != 1) || first_step < 0) return GPUE_EINVAL;
+  for (int s = 0; s < nsteps;) {
+    const int L = block_len(c, first_step, nsteps, s, energy_every);
+    if (L == 0) {
+      ++s;
+      continue;
+    }
+    StepGraph* sg = nullptr;
+    gpue_status st = step_graph(c, gstate, L, &sg);
+    if (st != GPUE_OK) return st;
+    sg->last_use = ++c->graph_clock;
+    s += L;
+  }
+  return GPUE_OK;
 }
 
 }  // extern "C"

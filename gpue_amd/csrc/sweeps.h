@@ -299,12 +299,14 @@ struct Pt {
 // once at load.  Only the reductions stay in FP64 in the FP32 form.
 
 // psi * exp(-arg) (imaginary time) or psi * exp(-i arg) (real time), times s
-// X (FP32 real-time kernels only): the phase factor and the product in FP64,
-// rounded to FP32 once.  A factor rounded to FP32 carries a modulus error of
-// ~1e-7 that is the same at every step for static V, A and K, so the energy
-// error would grow linearly with the steps; the rounding of the product
-// depends on psi.  Separate kernels, so the imaginary-time ones keep their
-// registers.
+// X (FP32 real-time kernels only): the phase factor is formed in FP32, then its
+// modulus is corrected in FP64 by one Newton step towards 1/sqrt(c^2 + s^2),
+// and the product is formed in FP64 and rounded to FP32 once.  A factor
+// rounded to FP32 carries a modulus error of ~1e-7 that is the same at every
+// step for static V, A and K, so the norm and energy would drift linearly with
+// the steps; after the Newton step the modulus is 1 within ~1e-14, and what is
+// left (the FP32 phase, ~1e-7 relative) acts like a rounded V and does not grow.
+// Separate kernels, so the imaginary-time ones keep their registers.
 template <typename T, int E, bool X = false>
 __device__ __forceinline__ void apply_phase_all(cx<T>* v, const T (&arg)[E], int gstate, double s_) {
   const T s = (T)s_;
@@ -315,28 +317,28 @@ __device__ __forceinline__ void apply_phase_all(cx<T>* v, const T (&arg)[E], int
       v[r] = cscale(v[r], f);
     }
   } else if constexpr (X && sizeof(T) == 4) {
-    double a[E];
     bool small = true, fast = true;
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-      a[r] = -(double)arg[r];
-      small = small && sincos_small_ok(a[r]);
-      fast = fast && sincos_fast_ok(a[r]);
+      small = small && sincos_small_ok(-arg[r]);
+      fast = fast && sincos_fast_ok(-arg[r]);
     }
-    double sn[E], cs[E];
+    float sn[E], cs[E];
     if (small) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) sincos_poly(a[r], &sn[r], &cs[r]);
+      for (int r = 0; r < E; ++r) sincos_poly(-arg[r], &sn[r], &cs[r]);
     } else if (fast) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) sincos_core(a[r], &sn[r], &cs[r]);
+      for (int r = 0; r < E; ++r) sincos_core(-arg[r], &sn[r], &cs[r]);
     } else {
 #pragma unroll
-      for (int r = 0; r < E; ++r) sincos_cw(a[r], &sn[r], &cs[r]);
+      for (int r = 0; r < E; ++r) sincos_cw(-arg[r], &sn[r], &cs[r]);
     }
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-      const double x = v[r].x, y = v[r].y, c = cs[r] * s_, d = sn[r] * s_;
+      const double c0 = cs[r], d0 = sn[r];
+      const double f = fma(-0.5, fma(c0, c0, d0 * d0), 1.5) * s_;
+      const double c = c0 * f, d = d0 * f, x = v[r].x, y = v[r].y;
       v[r] = mk((float)(c * x - d * y), (float)(c * y + d * x));
     }
   } else {

@@ -117,7 +117,13 @@ Grid parse_args(int argc, char** argv) {
   par.store("ramp_type", 1);
   optind = 1;
   int opt;
-  while ((opt = getopt(argc, argv, "b:d:C:x:y:w:m:G:g:e:T:t:n:p:rQ:L:E::lsi:X:Y:WU:V:z:A:v:Z:fc:I:j:B:M:N:P:oR:")) != -1) {
+  // GPUE's short flags of the step (src/parser.cu:71-72) plus -B/-M/-N/-o, letters GPUE
+  // leaves unused; the working precision is a long option because every other letter
+  // is taken (GPUE's -P is the laser power of its optical-lattice experiments).
+  static const struct option longopts[] = {{"precision", required_argument, nullptr, 1000},
+                                           {nullptr, 0, nullptr, 0}};
+  while ((opt = getopt_long(argc, argv, "b:d:C:x:y:w:m:G:g:e:T:t:n:p:rQ:L:E::lsi:X:Y:WU:V:z:A:v:Z:fc:I:j:B:M:N:oR:",
+                            longopts, nullptr)) != -1) {
     switch (opt) {
       case 'x': par.store("xDim", std::atoi(optarg)); break;
       case 'y': par.store("yDim", std::atoi(optarg)); break;
@@ -175,7 +181,12 @@ Grid parse_args(int argc, char** argv) {
       case 'B': par.store("hbar", std::atof(optarg)); break;
       case 'M': par.store("mass", std::atof(optarg)); break;
       case 'N': par.store("g_direct", std::atof(optarg)); break;
-      case 'P': par.store("precision", std::atoi(optarg)); break;
+      case 1000: {
+        const int prec = std::atoi(optarg);
+        if (prec != 64 && prec != 32) throw std::runtime_error("--precision must be 64 or 32");
+        par.store("precision", prec);
+        break;
+      }
       case 'o': par.store("stale_handoff", false); break;
       case 'R':   // src/parser.cu:188-195
         std::printf("Ramping omega with imaginary time evolution\n");
@@ -183,6 +194,8 @@ Grid parse_args(int argc, char** argv) {
         par.store("ramp_type", std::atoi(optarg));
         break;
       default:
+        if (optopt == 0 && optind > 0 && optind <= argc)   // an unknown long option
+          throw std::runtime_error(std::string("unsupported option ") + argv[optind - 1]);
         throw std::runtime_error(std::string("unsupported option -") + (char)optopt +
                                  " (analysis/experiment flags of GPUE are outside the step)");
     }

@@ -152,10 +152,16 @@ gpue_status gpue_sync(gpue_ctx *c);
  * 2, 3).  gpue_set_variants forces a choice. */
 gpue_status gpue_autotune(gpue_ctx *c, int reps, int *variants);
 gpue_status gpue_set_variants(gpue_ctx *c, const int *variants);
-/* Steady-state steps are captured as a HIP graph of `steps` consecutive steps
- * (even, default 8) and replayed; 0 launches every sweep individually.  No
- * reference counterpart (GPUE launches each kernel from the host loop). */
+/* Steady-state steps are captured as HIP graphs of up to `steps` consecutive
+ * steps (even, default 8) and replayed; 0 launches every sweep individually.
+ * No reference counterpart (GPUE launches each kernel from the host loop). */
 gpue_status gpue_set_graph_steps(gpue_ctx *c, int steps);
+/* Capture, without launching, the step blocks that gpue_evolve with the same
+ * first_step, nsteps, gstate and energy_every would replay (blocks of up to
+ * `steps` steps; shorter ones end the call and precede each energy
+ * evaluation), so that call spends no time on captures.  No reference
+ * counterpart. */
+gpue_status gpue_prepare_evolve(gpue_ctx *c, int first_step, int nsteps, int gstate, int energy_every);
 
 /* --- measurement hooks (no reference counterpart: GPUE prints clock() deltas,
  * src/evolution.cu:436-441) --- */

@@ -357,6 +357,9 @@ void or_get_grid(const or_ctx *c, int axis, double *out) {
 void or_set_wfc(or_ctx *c, const double *v) { memcpy(c->wfc, v, sizeof(or_c) * (size_t)c->N); }
 void or_get_wfc(const or_ctx *c, double *v) { memcpy(v, c->wfc, sizeof(or_c) * (size_t)c->N); }
 void or_set_g(or_ctx *c, double g) { c->p.g = g; }
+/* OpenMP threads of the following calls (the CPU baseline runs at all cores and at 1) */
+void or_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+int or_max_threads(void) { return omp_get_max_threads(); }
 
 /* ------------------------------------------------------------------------ */
 /* Pointwise kernels restated                                                */

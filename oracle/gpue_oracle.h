@@ -66,6 +66,8 @@ void    or_get_grid(const or_ctx *c, int axis, double *out);      /* 0..2 x,y,z 
 void    or_set_wfc(or_ctx *c, const double *interleaved);
 void    or_get_wfc(const or_ctx *c, double *interleaved);
 void    or_set_g(or_ctx *c, double g);
+void    or_set_threads(int n);     /* OpenMP threads of the following calls */
+int     or_max_threads(void);
 double  or_dV(const or_ctx *c);
 
 /* --- hot path --- */

@@ -62,6 +62,8 @@ def lib():
         L.or_set_wfc.argtypes = [vp, dp]
         L.or_get_wfc.argtypes = [vp, dp]
         L.or_set_g.argtypes = [vp, ctypes.c_double]
+        L.or_set_threads.argtypes = [ctypes.c_int]
+        L.or_max_threads.restype = ctypes.c_int
         L.or_dV.argtypes = [vp]
         L.or_dV.restype = ctypes.c_double
         L.or_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -247,6 +249,15 @@ class Oracle:
         done = lib().or_evolve(self._h, nsteps, gstate, energy_every, thr, _dp(en), max_energies,
                                ctypes.byref(ne))
         return done, en[: min(ne.value, max_energies)]
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's following calls."""
+    lib().or_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().or_max_threads())
 
 
 def ramp_factor(ramp_type, omega, i, num_steps):

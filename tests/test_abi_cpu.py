@@ -80,3 +80,21 @@ def test_eqn_parse_rejects_bad_buffers():
     assert lib.gpue_eqn_parse(b"x*x+y", None, None, 0, big, need, ctypes.byref(nn), ctypes.byref(td)) == G.GPUE_OK
     assert lib.gpue_eqn_eval(big, need, 2.0, 3.0, 0.0, 0.0) == 7.0
     assert lib.gpue_step_bytes(None, 0, None, None) == G.GPUE_EINVAL
+
+
+GPUE_BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpue_amd", "bin", "gpue")
+
+
+@pytest.mark.parametrize("args,message", [
+    (["-P", "5"], "unsupported option -P"),            # GPUE's laser power (src/parser.cu:295-299)
+    (["-q", "1"], "unsupported option -q"),
+    (["--precision", "16"], "--precision must be 64 or 32"),
+    (["--precison", "32"], "unsupported option --precison"),
+])
+def test_cli_rejects_flags_outside_the_step(tmp_path, args, message):
+    # parse_args fails before any device call, so this runs without a GPU
+    import subprocess
+
+    r = subprocess.run([GPUE_BIN, "-d", str(tmp_path)] + args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert message in r.stderr

@@ -354,3 +354,61 @@ def test_autotune_leaves_state(prec):
     assert a.energy() == b.energy()
     a.close()
     b.close()
+
+
+def test_fp32_real_time_long_run_does_not_drift():
+    # FP32-bulk real time over 2000 steps (graphs on) against the FP64 path: the
+    # relative energy and mu errors stay within the FP32 bar, and the error after
+    # 2000 steps is far below 100x the error after 20 steps (a phase factor whose
+    # modulus error repeats every step would make it grow linearly)
+    kw = dict(dimnum=2, xDim=128, yDim=128, xMax=10, yMax=10, g=200, gpe=1, corotating=1, omega=0.7, gdt=1e-3,
+              dt=5e-4)
+    a, b = G.Context(precision=64, **kw), G.Context(precision=32, **kw)
+    psi = seeded_psi(a.shape, [a.grid(0), a.grid(1)], seed=3)
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * (a.grid(0)[1] - a.grid(0)[0]) ** 2)
+    errs = {}
+    for c in (a, b):
+        c.set_graph_steps(32)
+        c.upload(psi)
+    done = 0
+    for target in (20, 2000):
+        for c in (a, b):
+            c.evolve(target - done, 1, first_step=done)
+        done = target
+        (ea, ma), (eb, mb) = a.energy(), b.energy()
+        errs[target] = (abs(eb - ea) / abs(ea), abs(mb - ma) / abs(ma))
+        print(target, errs[target])
+        assert errs[target][0] <= TOL32_E and errs[target][1] <= TOL32_E, (target, errs[target])
+    floor = 1e-8
+    assert errs[2000][0] <= 30 * max(errs[20][0], floor), errs
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("change", ["set_g", "set_variants", "set_graph_steps"])
+def test_parameter_change_right_after_graph_replays(change):
+    # a real-time evolve returns with graph replays still queued; changing a
+    # parameter destroys the captured graphs, which must wait for them.  The
+    # result equals a run that synchronised before the change.
+    kw = dict(dimnum=2, xDim=64, yDim=64, xMax=8, yMax=8, g=80, gpe=1, corotating=1, omega=0.5, gdt=1e-3, dt=5e-4)
+    psi = None
+    out = []
+    for sync_first in (False, True):
+        c = G.Context(**kw)
+        c.set_graph_steps(8)
+        if psi is None:
+            psi = seeded_psi(c.shape, [c.grid(0), c.grid(1)], seed=7, phase=True)
+        c.upload(psi)
+        c.evolve(64, 1)
+        if sync_first:
+            c.sync()
+        if change == "set_g":
+            c.set_g(90.0)
+        elif change == "set_variants":
+            c.set_variants([1, 1, 1])
+        else:
+            c.set_graph_steps(4)
+        c.evolve(16, 1, first_step=64)
+        out.append(c.download())
+        c.close()
+    assert rel_l2(out[0], out[1]) <= 1e-13
