@@ -103,8 +103,8 @@ MEASURED_TRAFFIC = {
 }
 
 RM = {"row_fwd": 0, "row_inv": 1, "row_gauge": 2, "row_full": 3, "row_ksum": 4, "row_inv_fwd": 5,
-      "row_gauge_fwd": 6}
-CM = {"fwd": 0, "inv": 1, "k": 2, "gauge": 3}
+      "row_gauge_fwd": 6, "row_k": 7}
+CM = {"fwd": 0, "inv": 1, "k": 2, "gauge": 3, "ksum": 4, "fwdv": 5, "invvg": 6, "gauge_fwd": 7}
 
 
 def kernel_symbol(name, prec, n_axis, variant):
@@ -299,6 +299,14 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
     for c in ctxs:
         c.close()
     conv = sum(1 for d in done_at if d is not None)
+    # convergence against the rotation rate: rotating instances nucleate vortices
+    # and relax them slowly in imaginary time
+    by_omega = {}
+    for lo, hi in ((0.0, 0.3), (0.3, 0.6), (0.6, 0.95)):
+        ks = [k for k, (cfg, _) in enumerate(specs) if lo <= cfg["omega"] < hi or (hi == 0.95 and cfg["omega"] >= hi)]
+        ch = [last_change[k] for k in ks if last_change[k] is not None]
+        by_omega[f"{lo}-{hi}"] = {"instances": len(ks), "converged": sum(1 for k in ks if done_at[k] is not None),
+                                  "median_last_rel_change_per_100_steps": float(np.median(ch)) if ch else None}
     return {"instances": n_inst, "sizes": {int(n): int(np.sum(sizes == n)) for n in sorted(set(sizes))},
             "converged": conv, "max_steps": max_steps, "wall_s": round(el, 3),
             "instance_steps_per_s": round(sum(steps) / el, 1),
@@ -307,6 +315,7 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
             "variants_by_size": {int(k): list(v) for k, v in tuned.items()},
             "batch_tuning_us_per_batch_step": {k: round(v * 1e6, 1) for k, v in trial.items()},
             "steps_to_converge": {"min": int(min(steps)), "max": int(max(steps))},
+            "convergence_by_omega": by_omega,
             "last_rel_energy_change_per_100_steps": {
                 "median": float(np.median([x for x in last_change if x is not None])),
                 "max": float(max(x for x in last_change if x is not None))}}

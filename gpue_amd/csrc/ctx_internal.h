@@ -43,6 +43,7 @@ struct gpue_ctx {
   double d[3] = {1, 1, 1};
   std::vector<double> grid[6];
   hipStream_t stream = nullptr;
+  bool own_stream = true;      // false: a stream the serving loop shares among its requests
   void* psi = nullptr;
   void* tmp = nullptr;
   double* dgrid[6] = {};
@@ -55,6 +56,7 @@ struct gpue_ctx {
   // steps between energy evaluations
   int graph_steps = 8;
   std::vector<gpue::StepGraph> graphs;
+  hipEvent_t gev = nullptr;    // recorded after every graph replay (a shared stream is not waited for)
   long graph_clock = 0;        // use counter for the least-recently-used eviction
   double* partials = nullptr;
   size_t partials_n = 0;
@@ -70,10 +72,15 @@ struct gpue_ctx {
   double* Kt = nullptr;
   double* At[3] = {};
   bool az_zero = true;
-  bool v_time = false;         // V is a time-dependent equation, refreshed each step
+  // time-dependent equation fields (V, K, Ax, Ay, Az): device programs evaluated
+  // per point inside the sweeps; tables of the same fields are filled at time
+  // t_last only for the energy
+  gpue::EqnProg* deq[5] = {};
+  bool t_dep = false;          // some field depends on time (steps run one by one)
+  double t_last = 0.0;         // time of the last step, Dt * i
+  double* vst[3] = {};         // the harmonic trap's per-axis terms (V_TRAP_A)
   int ramp_type = 0;           // -R: gauge exponent scaled per step (0 off, 1, 2)
   int ramp_steps = 0;          // numSteps of the ramped evolve call
-  gpue::EqnProg veq;
   bool pending_norm = false;   // results[0] holds a norm not yet applied (else it holds 1.0)
   int next_step = 0;           // step index after the last evolve (batch parity)
   int defer_nb = 0;            // eager steps: a finalize left to the next step's first sweep
@@ -109,5 +116,12 @@ gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void* in, 
 gpue_status apply_pending(gpue_ctx* c);
 gpue_status evolve_async(gpue_ctx* c, int first_step, int nsteps, int gstate);
 gpue_status energy_launch(gpue_ctx* c, double* host_dst);
+// psi to the host, waiting on `ev` (recorded after the copy) rather than the stream.
+gpue_status download_wfc_ev(gpue_ctx* c, double* interleaved, hipEvent_t ev);
+// A context on a given stream (null: its own), and new parameters for one of the
+// same shape (no device allocation).
+gpue_status ctx_create(const gpue_params* p, hipStream_t ext_stream, gpue_ctx** out);
+gpue_status ctx_reset(gpue_ctx* c, const gpue_params* p);
+bool ctx_same_shape(const gpue_ctx* c, const gpue_params* p);
 void energy_combine(const gpue_ctx* c, const double* terms, double* energy, double* mu);
 }  // namespace gpue

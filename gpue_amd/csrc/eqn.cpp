@@ -27,6 +27,8 @@
 
 #include "eqn.h"
 
+#include <algorithm>
+
 namespace gpue {
 
 namespace {
@@ -281,6 +283,12 @@ int eqn_to_program(const EqnNodeC* nodes, int n, EqnProgH& prog, std::string& er
   try {
     prog.n = 0;
     if (n > 0) to_postfix(nodes, 0, prog);
+    int sp = 0;
+    prog.depth = 0;
+    for (int i = 0; i < prog.n; ++i) {
+      sp += prog.op[i] < EQH_OP ? 1 : -1;
+      prog.depth = std::max(prog.depth, sp);
+    }
     return 0;
   } catch (const std::exception& ex) {
     err = ex.what();

@@ -21,6 +21,7 @@ enum { EQH_CONST = 0, EQH_X = 1, EQH_Y = 2, EQH_Z = 3, EQH_T = 4, EQH_OP = 16 };
 constexpr int EQ_MAX_H = 48;
 struct EqnProgH {
   int n;
+  int depth;   // largest stack depth the program reaches
   int op[EQ_MAX_H];
   double val[EQ_MAX_H];
 };

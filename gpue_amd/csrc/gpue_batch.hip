@@ -34,6 +34,7 @@ struct BSeq {                       // one launch sequence of a group for (gstat
   std::vector<std::vector<SweepArgs>> hargs;
   FinArgs* dfin = nullptr;          // imaginary time: device array [members]
   std::vector<FinArgs> hfin;
+  bool any = true;                  // some member takes this sequence (else it is not launched)
 };
 
 struct Group {
@@ -86,17 +87,22 @@ static gpue_status bfail(gpue_batch* b, gpue_status s, const std::string& m) {
     if (e_ != hipSuccess) return bfail((b), GPUE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Fill the host argument arrays of every sequence from the members' current state.
-static void fill_args(gpue_batch* b) {
+// Fill the host argument arrays of every sequence from the members' current
+// state; returns true when the set of sequences that are launched changed
+// (captured graphs hold the launches, so they are captured again).
+static bool fill_args(gpue_batch* b) {
+  bool pattern = false;
   for (Group& g : b->groups)
     for (int gs = 0; gs < 2; ++gs)
       for (int ph = 0; ph < 2; ++ph)
-        for (BSeq& s : g.seqs[gs][ph])
+        for (BSeq& s : g.seqs[gs][ph]) {
+          bool any = false;
           for (size_t m = 0; m < g.members.size(); ++m) {
             const int idx = g.members[m];
             gpue_ctx* c = b->ctx[idx];
             const int par = (b->cls[idx] + ph) % 2;   // the instance's step parity at this phase
             const bool in = b->active[idx] && ((s.mask >> par) & 1);
+            any = any || in;
             const std::vector<Sweep>& sc = g.sched[gs][par];
             const Phys phys = make_phys(c, gs, 0.0);
             for (size_t k = 0; k < s.keys.size(); ++k) {
@@ -137,6 +143,10 @@ static void fill_args(gpue_batch* b) {
               f.out = c->results;
             }
           }
+          if (any != s.any) pattern = true;
+          s.any = any;
+        }
+  return pattern;
 }
 
 static gpue_status upload_args(gpue_batch* b) {
@@ -158,6 +168,7 @@ static gpue_status launch_group_phase(gpue_batch* b, Group& g, int gs, int ph) {
   {
     gpue_ctx* c = g.proto;
     for (BSeq& s : g.seqs[gs][ph]) {
+      if (!s.any) continue;   // the step parity no member has
       for (size_t k = 0; k < s.keys.size(); ++k) {
         const Sweep& sw = s.keys[k];
         const int n = c->dims[sw.axis];
@@ -165,7 +176,7 @@ static gpue_status launch_group_phase(gpue_batch* b, Group& g, int gs, int ph) {
         const int others = sw.row ? 1 : (sw.axis == 1 ? c->dims[2] : c->dims[1]);
         bool handled = false;
         // FP32 real time: the kernels with FP64 phase factors, as the single-context path
-        const bool x = c->prec == 32 && gs != 0 && (sw.row || sw.mode == CM_K || sw.mode == CM_GAUGE);
+        const bool x = c->prec == 32 && gs != 0 && (sw.row || col_phase_mode(sw.mode));
         hipError_t e = launch_sweep_batched(c->prec, sw.row, n, c->var[sw.axis], x ? (sw.mode | 16) : sw.mode, s.dargs[k],
                                             (int)g.members.size(), lx, others, g.stream, &handled);
         if (!handled || e != hipSuccess) return bfail(b, GPUE_EHIP, "batched sweep launch");
@@ -232,7 +243,7 @@ gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
   gpue_batch* b = new gpue_batch();
   for (int i = 0; i < n; ++i) {
     gpue_ctx* c = ctxs[i];
-    if (!c || c->v_time || c->ramp_type || c->p.device != ctxs[0]->p.device) {
+    if (!c || c->t_dep || c->ramp_type || c->p.device != ctxs[0]->p.device) {
       gpue_batch_destroy(b);
       return GPUE_EINVAL;   // time-dependent V or ramp and mixed devices are run per context
     }
@@ -317,8 +328,11 @@ gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
 gpue_status gpue_batch_set_active(gpue_batch* b, int index, int active) {
   if (!b || index < 0 || index >= (int)b->ctx.size()) return GPUE_EINVAL;
   b->active[index] = active ? 1 : 0;
-  fill_args(b);
-  return upload_args(b);   // graphs keep their pointers; only the contents change
+  // graphs keep their argument pointers, only the contents change, unless a
+  // sequence starts or stops being launched (every evolve call ends synchronised,
+  // so no replay is in flight here)
+  if (fill_args(b)) drop_batch_graphs(b);
+  return upload_args(b);
 }
 
 // Advance every active instance by nsteps steps of gstate.
@@ -340,7 +354,7 @@ gpue_status gpue_batch_evolve(gpue_batch* b, int nsteps, int gstate) {
     }
   }
   if (changed) {
-    fill_args(b);
+    if (fill_args(b)) drop_batch_graphs(b);
     gpue_status st = upload_args(b);
     if (st != GPUE_OK) return st;
   }

@@ -82,15 +82,6 @@ static double* dev_copy(const std::vector<double>& v, hipError_t* e) {
   return p;
 }
 
-static float* dev_copy_f(const std::vector<double>& v, hipError_t* e) {
-  std::vector<float> h(v.begin(), v.end());   // rounded once, to nearest
-  float* p = nullptr;
-  *e = hipMalloc((void**)&p, std::max<size_t>(1, h.size()) * sizeof(float));
-  if (*e != hipSuccess) return nullptr;
-  *e = hipMemcpy(p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
-  return p;
-}
-
 Phys gpue::make_phys(const gpue_ctx* c, int gstate, double time) {
   const gpue_params& p = c->p;
   Phys f;
@@ -120,7 +111,6 @@ Phys gpue::make_phys(const gpue_ctx* c, int gstate, double time) {
   f.x0 = p.x0_shift;
   f.y0 = p.y0_shift;
   f.z0e = p.dimnum == 3 ? p.z0_shift : 0.0;     // items[8]
-  (void)time;
   for (int a = 0; a < 3; ++a) {
     (&f.xg)[a] = c->dgrid[a];
     (&f.pxg)[a] = c->dgrid[a + 3];
@@ -133,6 +123,17 @@ Phys gpue::make_phys(const gpue_ctx* c, int gstate, double time) {
   f.Vt = c->Vt;
   f.Kt = c->Kt;
   for (int a = 0; a < 3; ++a) f.At[a] = c->At[a];
+  for (int k = 0; k < 5; ++k) f.eq[k] = c->deq[k];
+  for (int a = 0; a < 3; ++a) {
+    f.vst[a] = c->vst[a];
+    f.dxyz[a] = c->d[a];
+  }
+  // find_field's coordinates dx*i - xMax (src/dynamic.cu:613-615)
+  f.org[0] = -p.xMax;
+  f.org[1] = p.dimnum > 1 ? -p.yMax : 0.0;
+  f.org[2] = p.dimnum > 2 ? -p.zMax : 0.0;
+  f.t = time;
+  f.eqk = c->t_dep ? 1 : 0;
   return f;
 }
 
@@ -149,48 +150,75 @@ double gpue::ramp_scale(const gpue_ctx* c, int step) {
   return w * (double)(step + 1) / (double)c->ramp_steps;
 }
 
-// Sweeps of step i (src/evolution.cu:727-870).  NORM is set on the last sweep
-// of an imaginary-time step; SCALE_IN on the first when a norm is pending.
+// Sweeps of step i (src/evolution.cu:727-870): V/2, the forward transforms, K,
+// the inverse transforms, V/2, then the gauge operators in GPUE's order (x -> y
+// (-> z) on odd steps, (z ->) y -> x on even ones, src/evolution.cu:837,842).
+// The transforms of different axes commute, so the step is led by the axis
+// whose gauge operator comes first: its closing sweep [IFFT, V/2, FFT, A, IFFT]
+// then applies that operator too, and a 2D gauge step takes 4 sweeps in either
+// order (3D: 6, or 7 with a non-zero Az) instead of 5 when the order starts on
+// another axis.  The K step runs on the innermost axis.  NORM is set on the
+// last sweep of an imaginary-time step; SCALE_IN on the first when a norm is
+// pending.
 void gpue::schedule(const gpue_ctx* c, int gstate, int i, bool pending, std::vector<Sweep>& out) {
   out.clear();
   const gpue_params& p = c->p;
-  const int nx = c->dims[0], ny = c->dims[1], nz = c->dims[2];
+  const int n[3] = {c->dims[0], c->dims[1], c->dims[2]};
   const double invN = 1.0 / (double)c->N;
   const unsigned scale = pending ? F_SCALE_IN : 0u;
   const bool flip = (i % 2) != 0;
   if (p.dimnum == 1) {
     unsigned f = scale | F_HALFV_PRE | F_HALFV_POST | (p.corotating ? F_GAUGE_POST : 0u);
     out.push_back({true, 0, RM_FULL, f, invN});
-  } else {
-    out.push_back({true, 0, RM_FWD, scale | F_HALFV_PRE, 0.0});
-    if (p.dimnum == 2) {
-      out.push_back({false, 1, CM_K, 0u, invN});
-    } else {
-      out.push_back({false, 1, CM_FWD, 0u, 0.0});
-      out.push_back({false, 2, CM_K, 0u, invN});
-      out.push_back({false, 1, CM_INV, 0u, 0.0});
-    }
+    if (gstate == 0) out.back().flags |= F_NORM;
+    return;
+  }
+  // gauge axes in GPUE's order for this step (the z operator is the identity when Az = 0)
+  int G[3], ng = 0;
+  if (p.corotating) {
     const bool zg = p.dimnum == 3 && !c->az_zero;
-    if (!p.corotating) {
-      out.push_back({true, 0, RM_INV, F_HALFV_POST, 0.0});
-    } else if (flip) {   // x -> y (-> z)
-      out.push_back({true, 0, RM_INV, F_HALFV_POST | F_GAUGE_POST, 1.0 / nx});
-      out.push_back({false, 1, CM_GAUGE, 0u, 1.0 / ny});
-      if (zg) out.push_back({false, 2, CM_GAUGE, 0u, 1.0 / nz});
-    } else {             // (z ->) y -> x
-      out.push_back({true, 0, RM_INV, F_HALFV_POST, 0.0});
-      if (zg) out.push_back({false, 2, CM_GAUGE, 0u, 1.0 / nz});
-      out.push_back({false, 1, CM_GAUGE, 0u, 1.0 / ny});
-      out.push_back({true, 0, RM_GAUGE, 0u, 1.0 / nx});
+    if (flip) {
+      G[ng++] = 0;
+      G[ng++] = 1;
+      if (zg) G[ng++] = 2;
+    } else {
+      if (zg) G[ng++] = 2;
+      G[ng++] = 1;
+      G[ng++] = 0;
     }
   }
+  const int lead = ng ? G[0] : 0;
+  int order[3], no = 0;   // forward transform order: the lead axis first, the K axis last
+  order[no++] = lead;
+  for (int a = 0; a < p.dimnum; ++a)
+    if (a != lead) order[no++] = a;
+  auto fwd = [&](int a, unsigned f) -> Sweep {
+    return a == 0 ? Sweep{true, 0, RM_FWD, f, 0.0} : Sweep{false, a, f ? (int)CM_FWD_V : (int)CM_FWD, f, 0.0};
+  };
+  auto inv = [&](int a) -> Sweep { return a == 0 ? Sweep{true, 0, RM_INV, 0u, 0.0} : Sweep{false, a, CM_INV, 0u, 0.0}; };
+  out.push_back(fwd(lead, scale | F_HALFV_PRE));
+  for (int k = 1; k + 1 < no; ++k) out.push_back(fwd(order[k], 0u));
+  const int kax = order[no - 1];
+  out.push_back(kax == 0 ? Sweep{true, 0, RM_K, 0u, invN} : Sweep{false, kax, CM_K, 0u, invN});
+  for (int k = no - 2; k >= 1; --k) out.push_back(inv(order[k]));
+  // closing sweep of the lead axis: IFFT, V/2 and, with a gauge, that axis' operator
+  const unsigned fc = F_HALFV_POST | (ng ? F_GAUGE_POST : 0u);
+  const double invc = ng ? 1.0 / n[lead] : 0.0;
+  out.push_back(lead == 0 ? Sweep{true, 0, RM_INV, fc, invc} : Sweep{false, lead, CM_INV_VG, fc, invc});
+  for (int k = 1; k < ng; ++k)
+    out.push_back(G[k] == 0 ? Sweep{true, 0, RM_GAUGE, 0u, 1.0 / n[0]} : Sweep{false, G[k], CM_GAUGE, 0u, 1.0 / n[G[k]]});
   if (gstate == 0) out.back().flags |= F_NORM;
+}
+
+// The sweep that opens a step and may rescale its input by a pending norm.
+static bool opens_step(const Sweep& s) {
+  return s.row ? s.mode == RM_FWD : s.mode == CM_FWD_V;
 }
 
 static std::string sweep_name(const Sweep& s) {
   static const char* rm[] = {"row_fwd", "row_inv", "row_gauge", "row_full", "row_ksum", "row_inv_fwd",
-                             "row_gauge_fwd"};
-  static const char* cm[] = {"fwd", "inv", "k", "gauge", "ksum"};
+                             "row_gauge_fwd", "row_k"};
+  static const char* cm[] = {"fwd", "inv", "k", "gauge", "ksum", "fwdv", "invvg", "gauge_fwd"};
   std::string n = s.row ? rm[s.mode] : (std::string(s.axis == 1 ? "coly_" : "colz_") + cm[s.mode]);
   if (s.flags & F_HALFV_PRE) n += "+vpre";
   if (s.flags & F_HALFV_POST) n += "+vpost";
@@ -222,9 +250,10 @@ static gpue_status timing_flush(gpue_ctx* c) {
 }
 
 // Workgroups (= partials written) of a sweep.
+static int default_variant(const gpue_ctx* c, int axis);
 static int sweep_nb(const gpue_ctx* c, const Sweep& s) {
   const int n = c->dims[s.axis];
-  const int var = s.var >= 0 ? s.var : c->var[s.axis];
+  const int var = c->t_dep ? default_variant(c, s.axis) : s.var >= 0 ? s.var : c->var[s.axis];
   return s.row ? sweep_blocks(c->prec, true, n, var, c->dims[1] * c->dims[2], 1)
                : sweep_blocks(c->prec, false, n, var, c->dims[0], s.axis == 1 ? c->dims[2] : c->dims[1]);
 }
@@ -236,8 +265,7 @@ static int sweep_nb(const gpue_ctx* c, const Sweep& s) {
 // for fewer than 4096 partials (every workgroup of `next` sums all of them)
 // and when `next` is the row sweep that scales its input.
 static bool defer_norm(const gpue_ctx* c, Sweep& last, Sweep& next) {
-  if (c->p.dimnum < 2 || !(last.flags & F_NORM) || !(next.flags & F_SCALE_IN) || !next.row || next.mode != RM_FWD)
-    return false;
+  if (c->p.dimnum < 2 || !(last.flags & F_NORM) || !(next.flags & F_SCALE_IN) || !opens_step(next)) return false;
   const int nb = sweep_nb(c, last);
   if (nb >= 4096) return false;
   last.flags |= F_DEFER;
@@ -256,9 +284,19 @@ static gpue_status settle_deferred(gpue_ctx* c) {
   return GPUE_OK;
 }
 
+// The variants every context starts with (no timing, so results never depend on
+// it): for small, latency-bound grids narrow lines with up-front twiddle loads
+// and quarter strided tiles (the autotuner's choice for 2D grids of
+// 128^2-512^2), wide lines otherwise.  Contexts with time-dependent fields
+// always run these (their EQ kernels exist for these variants only).
+static int default_variant(const gpue_ctx* c, int axis) {
+  const bool small = c->N <= (1L << 20);
+  return axis == 0 ? (small ? 3 : 0) : (small ? 5 : 0);
+}
+
 gpue_status gpue::launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void* in, void* out) {
   const int n = c->dims[s.axis];
-  const int var = s.var >= 0 ? s.var : c->var[s.axis];
+  const int var = c->t_dep ? default_variant(c, s.axis) : s.var >= 0 ? s.var : c->var[s.axis];
   SweepArgs a;
   std::memset(&a, 0, sizeof(a));
   a.active = 1;
@@ -326,7 +364,8 @@ static bool valid_dim(int n) { return sweep_supported(n); }
 // time), so the stream is drained before an executable graph is destroyed.
 static void drop_graphs(gpue_ctx* c) {
   if (c->graphs.empty()) return;
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->gev) (void)hipEventSynchronize(c->gev);   // the last replay (graph launches record it)
+  else if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (StepGraph& g : c->graphs) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -425,7 +464,17 @@ const char* gpue_last_error(const gpue_ctx* c) { return c ? c->err.c_str() : "nu
 
 void gpue_destroy(gpue_ctx* c) {
   if (!c) return;
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream && c->own_stream) {
+    (void)hipStreamSynchronize(c->stream);
+  } else if (c->stream) {   // a shared stream: wait for the work queued so far, not for later work
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
+      if (hipEventRecord(ev, c->stream) == hipSuccess) (void)hipEventSynchronize(ev);
+      (void)hipEventDestroy(ev);
+    } else {
+      (void)hipStreamSynchronize(c->stream);
+    }
+  }
   void* bufs[] = {c->psi, c->tmp, c->partials, c->results, c->Vt, c->Kt, c->At[0], c->At[1], c->At[2],
                   c->vs[0], c->vs[1], c->vs[2], c->ks[0], c->ks[1], c->ks[2], c->kf[0][0], c->kf[0][1],
                   c->kf[0][2], c->kf[1][0], c->kf[1][1], c->kf[1][2], c->vsf[0], c->vsf[1], c->vsf[2],
@@ -435,66 +484,45 @@ void gpue_destroy(gpue_ctx* c) {
     if (b) (void)hipFree(b);
   for (int a = 0; a < 6; ++a)
     if (c->dgrid[a]) (void)hipFree(c->dgrid[a]);
+  for (int k = 0; k < 5; ++k)
+    if (c->deq[k]) (void)hipFree(c->deq[k]);
+  for (int a = 0; a < 3; ++a)
+    if (c->vst[a]) (void)hipFree(c->vst[a]);
   for (auto& kv : c->tw) (void)hipFree(kv.second);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   drop_graphs(c);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->gev) (void)hipEventDestroy(c->gev);
+  if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
-gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
-  if (!pin || !out) return GPUE_EINVAL;
-  *out = nullptr;
-  gpue_ctx* c = new gpue_ctx();
-  c->p = *pin;
-  gpue_params& p = c->p;
-  auto bail = [&](gpue_status s, const std::string& m) {
-    c->err = m;
-    static thread_local std::string last;
-    last = m;
-    std::fprintf(stderr, "gpue_create: %s\n", m.c_str());
-    gpue_destroy(c);
-    return s;
-  };
-  if (p.dimnum < 1 || p.dimnum > 3) return bail(GPUE_EINVAL, "dimnum must be 1, 2 or 3");
-  if (p.precision != 64 && p.precision != 32) return bail(GPUE_EINVAL, "precision must be 64 or 32");
-  if (p.dimnum < 3) p.zDim = 1;
-  if (p.dimnum < 2) p.yDim = 1;
-  c->dims[0] = p.xDim;
-  c->dims[1] = p.yDim;
-  c->dims[2] = p.zDim;
-  for (int a = 0; a < p.dimnum; ++a)
-    if (!valid_dim(c->dims[a]))
-      return bail(GPUE_EUNSUPPORTED, "unsupported grid size " + std::to_string(c->dims[a]) +
-                                         " (supported: 16 32 48 64 96 128 192 256 384 512 1024)");
-  if (!(p.hbar > 0) || !(p.mass > 0) || !(p.xMax > 0) || (p.dimnum > 1 && !(p.yMax > 0)) ||
-      (p.dimnum > 2 && !(p.zMax > 0)))
-    return bail(GPUE_EINVAL, "hbar, mass and box sizes must be positive");
-  if (p.a_kind != GPUE_A_ROTATION && p.a_kind != GPUE_A_CONSTANT) return bail(GPUE_EINVAL, "bad a_kind");
-  c->prec = p.precision;
-  c->esz = p.precision == 64 ? 16 : 8;
-  c->N = (long)p.xDim * p.yDim * p.zDim;
+}  // extern "C"
+
+// Upload a host table into *slot (allocated on first use, then reused).
+template <typename TT>
+static hipError_t put_table(TT** slot, const std::vector<TT>& v) {
+  if (!*slot) {
+    hipError_t e = hipMalloc((void**)slot, std::max<size_t>(1, v.size()) * sizeof(TT));
+    if (e != hipSuccess) return e;
+  }
+  return hipMemcpy(*slot, v.data(), v.size() * sizeof(TT), hipMemcpyHostToDevice);
+}
+
+// The tables that follow from the physics parameters (grids, separable V and K,
+// the K operator factors and their FP32 copies), for a new context or one that
+// is given new parameters of the same shape.
+static gpue_status fill_tables(gpue_ctx* c) {
+  const gpue_params& p = c->p;
   c->a_kind = p.a_kind == GPUE_A_ROTATION ? A_ROTATION : A_ZERO;
   make_grids(p.dimnum, p.xDim, p.yDim, p.zDim, p.xMax, p.yMax, p.zMax, c->grid, c->d);
   std::vector<double> vsh[3], ksh[3];
   sep_tables(c, vsh, ksh);
-
-  hipError_t e = hipSetDevice(p.device);
-  if (e != hipSuccess) return bail(GPUE_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
-  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) return bail(GPUE_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-  if (hipMalloc(&c->psi, c->N * c->esz) != hipSuccess || hipMalloc(&c->tmp, c->N * c->esz) != hipSuccess)
-    return bail(GPUE_ENOMEM, "psi buffers");
-  for (int a = 0; a < 6; ++a) {
-    c->dgrid[a] = dev_copy(c->grid[a], &e);
-    if (e != hipSuccess) return bail(GPUE_ENOMEM, "grid arrays");
-  }
+  for (int a = 0; a < 6; ++a)
+    if (put_table(&c->dgrid[a], c->grid[a]) != hipSuccess) return fail(c, GPUE_ENOMEM, "grid arrays");
   for (int a = 0; a < 3; ++a) {
-    c->vs[a] = dev_copy(vsh[a], &e);
-    if (e != hipSuccess) return bail(GPUE_ENOMEM, "V tables");
-    c->ks[a] = dev_copy(ksh[a], &e);
-    if (e != hipSuccess) return bail(GPUE_ENOMEM, "K tables");
+    if (put_table(&c->vs[a], vsh[a]) != hipSuccess) return fail(c, GPUE_ENOMEM, "V tables");
+    if (put_table(&c->ks[a], ksh[a]) != hipSuccess) return fail(c, GPUE_ENOMEM, "K tables");
     // K operator factors: exp(-K_a gdt/hbar) and exp(-i K_a dt/hbar) (src/operators.cu:1082,1102-1103)
     std::vector<double> fg(2 * ksh[a].size()), fe(2 * ksh[a].size());
     for (size_t i = 0; i < ksh[a].size(); ++i) {
@@ -503,18 +531,113 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
       fe[2 * i] = std::cos(-ksh[a][i] * (p.dt / p.hbar));
       fe[2 * i + 1] = std::sin(-ksh[a][i] * (p.dt / p.hbar));
     }
-    c->kf[0][a] = dev_copy(fg, &e);
-    if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
-    c->kf[1][a] = dev_copy(fe, &e);
-    if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
-    if (c->prec == 32) {
-      c->vsf[a] = dev_copy_f(vsh[a], &e);
-      if (e != hipSuccess) return bail(GPUE_ENOMEM, "V tables");
-      c->kff[0][a] = dev_copy_f(fg, &e);
-      if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
-      c->kff[1][a] = dev_copy_f(fe, &e);
-      if (e != hipSuccess) return bail(GPUE_ENOMEM, "K factors");
+    if (put_table(&c->kf[0][a], fg) != hipSuccess || put_table(&c->kf[1][a], fe) != hipSuccess)
+      return fail(c, GPUE_ENOMEM, "K factors");
+    if (c->prec == 32) {   // rounded once, to nearest
+      const std::vector<float> v32(vsh[a].begin(), vsh[a].end()), g32(fg.begin(), fg.end()), e32(fe.begin(), fe.end());
+      if (put_table(&c->vsf[a], v32) != hipSuccess || put_table(&c->kff[0][a], g32) != hipSuccess ||
+          put_table(&c->kff[1][a], e32) != hipSuccess)
+        return fail(c, GPUE_ENOMEM, "FP32 tables");
     }
+  }
+  return GPUE_OK;
+}
+
+static gpue_status check_params(const gpue_params& p, std::string& msg) {
+  if (p.dimnum < 1 || p.dimnum > 3) return msg = "dimnum must be 1, 2 or 3", GPUE_EINVAL;
+  if (p.precision != 64 && p.precision != 32) return msg = "precision must be 64 or 32", GPUE_EINVAL;
+  const int dims[3] = {p.xDim, p.dimnum > 1 ? p.yDim : 1, p.dimnum > 2 ? p.zDim : 1};
+  for (int a = 0; a < p.dimnum; ++a)
+    if (!valid_dim(dims[a]))
+      return msg = "unsupported grid size " + std::to_string(dims[a]) +
+                   " (supported: 16 32 48 64 96 128 192 256 384 512 1024)",
+             GPUE_EUNSUPPORTED;
+  if (!(p.hbar > 0) || !(p.mass > 0) || !(p.xMax > 0) || (p.dimnum > 1 && !(p.yMax > 0)) ||
+      (p.dimnum > 2 && !(p.zMax > 0)))
+    return msg = "hbar, mass and box sizes must be positive", GPUE_EINVAL;
+  if (p.a_kind != GPUE_A_ROTATION && p.a_kind != GPUE_A_CONSTANT) return msg = "bad a_kind", GPUE_EINVAL;
+  return GPUE_OK;
+}
+
+bool gpue::ctx_same_shape(const gpue_ctx* c, const gpue_params* p) {
+  return p->dimnum == c->p.dimnum && p->precision == c->prec && p->device == c->p.device && p->xDim == c->dims[0] &&
+         (p->dimnum < 2 || p->yDim == c->dims[1]) && (p->dimnum < 3 || p->zDim == c->dims[2]);
+}
+
+static void clear_program(gpue_ctx* c, int k);
+
+// New parameters for a context of the same shape (grid sizes, precision,
+// device): the tables are refilled in place and every field returns to its
+// default, as for a new context; psi is left as it is.  No device memory is
+// allocated (the serving loop reuses finished requests' contexts this way).
+gpue_status gpue::ctx_reset(gpue_ctx* c, const gpue_params* pin) {
+  std::string msg;
+  gpue_status st = check_params(*pin, msg);
+  if (st != GPUE_OK) return fail(c, st, msg);
+  if (!ctx_same_shape(c, pin)) return fail(c, GPUE_EINVAL, "reset to another shape");
+  drop_graphs(c);   // waits for this context's last replay only
+  for (int k = 0; k < 5; ++k) clear_program(c, k);
+  gpue_params p = *pin;
+  if (p.dimnum < 3) p.zDim = 1;
+  if (p.dimnum < 2) p.yDim = 1;
+  c->p = p;
+  c->v_kind = V_SEP;
+  c->k_kind = K_SEP;
+  c->v_user = false;
+  c->az_zero = true;
+  c->ramp_type = 0;
+  c->pending_norm = false;
+  c->defer_nb = 0;
+  c->next_step = 0;
+  c->last_energy = 0.0;
+  c->t_last = 0.0;
+  c->vmask_valid = false;
+  st = fill_tables(c);
+  if (st != GPUE_OK) return st;
+  HIPCHK(c, launch_set_double(c->results, 1.0, c->stream));
+  return GPUE_OK;
+}
+
+gpue_status gpue::ctx_create(const gpue_params* pin, hipStream_t ext_stream, gpue_ctx** out) {
+  if (!pin || !out) return GPUE_EINVAL;
+  *out = nullptr;
+  gpue_ctx* c = new gpue_ctx();
+  c->p = *pin;
+  gpue_params& p = c->p;
+  auto bail = [&](gpue_status s, const std::string& m) {
+    c->err = m;
+    std::fprintf(stderr, "gpue_create: %s\n", m.c_str());
+    gpue_destroy(c);
+    return s;
+  };
+  {
+    std::string msg;
+    const gpue_status st = check_params(p, msg);
+    if (st != GPUE_OK) return bail(st, msg);
+  }
+  if (p.dimnum < 3) p.zDim = 1;
+  if (p.dimnum < 2) p.yDim = 1;
+  c->dims[0] = p.xDim;
+  c->dims[1] = p.yDim;
+  c->dims[2] = p.zDim;
+  c->prec = p.precision;
+  c->esz = p.precision == 64 ? 16 : 8;
+  c->N = (long)p.xDim * p.yDim * p.zDim;
+
+  hipError_t e = hipSetDevice(p.device);
+  if (e != hipSuccess) return bail(GPUE_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  if (ext_stream) {
+    c->stream = ext_stream;
+    c->own_stream = false;
+  } else {
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(GPUE_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  if (hipMalloc(&c->psi, c->N * c->esz) != hipSuccess || hipMalloc(&c->tmp, c->N * c->esz) != hipSuccess)
+    return bail(GPUE_ENOMEM, "psi buffers");
+  {
+    const gpue_status st = fill_tables(c);
+    if (st != GPUE_OK) return bail(st, c->err);
   }
   for (int a = 0; a < p.dimnum; ++a) {
     const int n = c->dims[a];
@@ -556,13 +679,7 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
     if (p.dimnum > 1) nb = std::max<size_t>(nb, sweep_blocks(c->prec, false, p.yDim, v, p.xDim, p.zDim));
     if (p.dimnum > 2) nb = std::max<size_t>(nb, sweep_blocks(c->prec, false, p.zDim, v, p.xDim, p.yDim));
   }
-  // default variants (no timing, so results never depend on it): for small,
-  // latency-bound grids narrow lines with up-front twiddle loads, and quarter
-  // strided tiles (the autotuner's choice for 2D grids of 128^2-512^2); wide
-  // lines otherwise
-  const bool small = c->N <= (1L << 20);
-  c->var[0] = small ? 3 : 0;
-  c->var[1] = c->var[2] = small ? 5 : 0;
+  for (int a = 0; a < 3; ++a) c->var[a] = default_variant(c, a);
   c->partials_n = nb * 4;
   if (hipMalloc((void**)&c->partials, c->partials_n * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->results, 16 * sizeof(double)) != hipSuccess)
@@ -576,6 +693,10 @@ gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) {
   *out = c;
   return GPUE_OK;
 }
+
+extern "C" {
+
+gpue_status gpue_create(const gpue_params* pin, gpue_ctx** out) { return ctx_create(pin, nullptr, out); }
 
 gpue_status gpue_get_grid(gpue_ctx* c, int axis, double* out) {
   if (!c || !out || axis < 0 || axis > 5) return GPUE_EINVAL;
@@ -596,6 +717,37 @@ gpue_status gpue_sync(gpue_ctx* c) {
   return GPUE_OK;
 }
 
+// The current A components as tables (the rotation gauge's or zero), before one
+// of them is replaced by a table or a time-dependent equation.
+static gpue_status materialise_A(gpue_ctx* c, int skip) {
+  if (c->a_kind == A_TABLE || c->a_kind == A_EQN) return GPUE_OK;
+  for (int a = 0; a < 3; ++a) {
+    if (a == skip) continue;
+    if (!c->At[a] && hipMalloc((void**)&c->At[a], c->N * sizeof(double)) != hipSuccess)
+      return fail(c, GPUE_ENOMEM, "field table");
+    std::vector<double> h(c->N, 0.0);
+    if (c->a_kind == A_ROTATION) {
+      for (long gid = 0; gid < c->N; ++gid) {
+        long xi = gid % c->dims[0], yi = (gid / c->dims[0]) % c->dims[1];
+        if (a == 0) h[gid] = -c->grid[1][yi] * c->p.omega * c->p.omegaX;
+        if (a == 1) h[gid] = c->grid[0][xi] * c->p.omega * c->p.omegaY;
+      }
+    }
+    HIPCHK(c, hipMemcpy(c->At[a], h.data(), c->N * sizeof(double), hipMemcpyHostToDevice));
+  }
+  c->a_kind = A_TABLE;
+  return GPUE_OK;
+}
+
+// Drop field k's time-dependent program (a static value replaces it).
+static void clear_program(gpue_ctx* c, int k) {
+  if (c->deq[k]) (void)hipFree(c->deq[k]);
+  c->deq[k] = nullptr;
+  c->t_dep = false;
+  for (int j = 0; j < 5; ++j) c->t_dep = c->t_dep || c->deq[j] != nullptr;
+  if (k >= 2 && !c->deq[2] && !c->deq[3] && !c->deq[4] && c->a_kind == A_EQN) c->a_kind = A_TABLE;
+}
+
 static gpue_status set_table(gpue_ctx* c, int which, const double* src, bool from_device) {
   drop_graphs(c);
   double** slot = which == GPUE_FIELD_V ? &c->Vt : which == GPUE_FIELD_K ? &c->Kt : &c->At[which - GPUE_FIELD_AX];
@@ -605,38 +757,29 @@ static gpue_status set_table(gpue_ctx* c, int which, const double* src, bool fro
   HIPCHK(c, hipMemcpyAsync(*slot, src, c->N * sizeof(double), from_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  clear_program(c, which);
   if (which == GPUE_FIELD_V) {
     c->v_kind = V_TABLE;
     c->v_user = true;
-    c->v_time = false;
   } else if (which == GPUE_FIELD_K) {
     c->k_kind = K_TABLE;
   } else {
     // the other components keep their current values: materialise them as tables
-    if (c->a_kind != A_TABLE) {
-      for (int a = 0; a < 3; ++a) {
-        if (a == which - GPUE_FIELD_AX) continue;
-        if (!c->At[a] && hipMalloc((void**)&c->At[a], c->N * sizeof(double)) != hipSuccess)
-          return fail(c, GPUE_ENOMEM, "field table");
-        std::vector<double> h(c->N, 0.0);
-        if (c->a_kind == A_ROTATION) {
-          for (long gid = 0; gid < c->N; ++gid) {
-            long xi = gid % c->dims[0], yi = (gid / c->dims[0]) % c->dims[1];
-            if (a == 0) h[gid] = -c->grid[1][yi] * c->p.omega * c->p.omegaX;
-            if (a == 1) h[gid] = c->grid[0][xi] * c->p.omega * c->p.omegaY;
-          }
-        }
-        HIPCHK(c, hipMemcpy(c->At[a], h.data(), c->N * sizeof(double), hipMemcpyHostToDevice));
-      }
-      c->a_kind = A_TABLE;
-    }
+    gpue_status st = materialise_A(c, which - GPUE_FIELD_AX);
+    if (st != GPUE_OK) return st;
     if (which == GPUE_FIELD_AZ) {
       std::vector<double> h(c->N);
       HIPCHK(c, hipMemcpy(h.data(), c->At[2], c->N * sizeof(double), hipMemcpyDeviceToHost));
       c->az_zero = std::all_of(h.begin(), h.end(), [](double v) { return v == 0.0; });
     }
     // GPUE builds V after A (src/operators.cu:726-829): a harmonic V follows the new A
-    if (!c->v_user) return materialise_V(c);
+    if (!c->v_user) {
+      if (c->a_kind == A_EQN) {
+        c->v_kind = V_TRAP_A;   // another component still depends on time
+        return GPUE_OK;
+      }
+      return materialise_V(c);
+    }
   }
   return GPUE_OK;
 }
@@ -669,14 +812,53 @@ gpue_status gpue_set_field_eqn(gpue_ctx* c, int which, const char* expr, const c
   const gpue_params& p = c->p;
   const double xM = p.xMax, yM = p.dimnum > 1 ? p.yMax : 0.0, zM = p.dimnum > 2 ? p.zMax : 0.0;
   if (td) {
-    if (which != GPUE_FIELD_V)
-      return fail(c, GPUE_EUNSUPPORTED, "time-dependent K and A fields are not supported (V only)");
-    if (!c->Vt && hipMalloc((void**)&c->Vt, c->N * sizeof(double)) != hipSuccess) return fail(c, GPUE_ENOMEM, "V");
+    // evaluated per point inside the step's sweeps at the step's time (the EQ
+    // kernels); the field's table is only filled for the energy
+    if (c->prec != 64)
+      return fail(c, GPUE_EUNSUPPORTED, "time-dependent equation fields need an FP64 context (GPUE's precision)");
+    if (dp.depth > EQ_REG_DEPTH)
+      return fail(c, GPUE_EUNSUPPORTED, "time-dependent equation needs a stack deeper than " +
+                                            std::to_string(EQ_REG_DEPTH));
     drop_graphs(c);
-    c->veq = dp;
-    c->v_kind = V_TABLE;
-    c->v_user = true;
-    c->v_time = true;
+    const int k = which;   // GPUE_FIELD_V, _K, _AX, _AY, _AZ = 0..4
+    double** slot = which == GPUE_FIELD_V ? &c->Vt : which == GPUE_FIELD_K ? &c->Kt : &c->At[which - GPUE_FIELD_AX];
+    if (!*slot && hipMalloc((void**)slot, c->N * sizeof(double)) != hipSuccess) return fail(c, GPUE_ENOMEM, "field");
+    if (which >= GPUE_FIELD_AX) {
+      gpue_status st = materialise_A(c, which - GPUE_FIELD_AX);
+      if (st != GPUE_OK) return st;
+    }
+    if (!c->deq[k] && hipMalloc((void**)&c->deq[k], sizeof(EqnProg)) != hipSuccess)
+      return fail(c, GPUE_ENOMEM, "equation program");
+    HIPCHK(c, hipMemcpy(c->deq[k], &dp, sizeof(EqnProg), hipMemcpyHostToDevice));
+    c->t_dep = true;
+    if (which == GPUE_FIELD_V) {
+      c->v_kind = V_EQN;
+      c->v_user = true;
+    } else if (which == GPUE_FIELD_K) {
+      c->k_kind = K_EQN;
+    } else {
+      c->a_kind = A_EQN;
+      if (which == GPUE_FIELD_AZ) c->az_zero = false;
+      if (!c->v_user) {   // V = trap + 1/2 m |A(t)|^2, per point
+        if (!c->Vt && hipMalloc((void**)&c->Vt, c->N * sizeof(double)) != hipSuccess)
+          return fail(c, GPUE_ENOMEM, "V");
+        for (int a = 0; a < 3 && !c->vst[a]; ++a) {
+          const gpue_params& q = c->p;
+          std::vector<double> t(c->grid[a].size());
+          const double oZ = q.dimnum == 3 ? q.omegaZ : 0.0, z0 = q.dimnum == 3 ? q.z0_shift : 0.0;
+          for (size_t i = 0; i < t.size(); ++i) {
+            const double x = c->grid[a][i];
+            const double u = a == 0 ? q.omegaX * (x + q.x0_shift)
+                                    : a == 1 ? q.gammaY * q.omegaY * (x + q.y0_shift) : 1.0 * oZ * (x + z0);
+            t[i] = 0.5 * q.mass * (u * u);
+          }
+          hipError_t e = hipSuccess;
+          c->vst[a] = dev_copy(t, &e);
+          if (e != hipSuccess) return fail(c, GPUE_ENOMEM, "trap tables");
+        }
+        c->v_kind = V_TRAP_A;
+      }
+    }
     return GPUE_OK;
   }
   // static: evaluate once on the device (find_field), then use as a table
@@ -724,6 +906,27 @@ gpue_status gpue_download_wfc(gpue_ctx* c, double* il) {
   return GPUE_OK;
 }
 
+}  // extern "C"
+
+gpue_status gpue::download_wfc_ev(gpue_ctx* c, double* il, hipEvent_t ev) {
+  gpue_status st = apply_pending(c);
+  if (st != GPUE_OK) return st;
+  if (c->prec == 64) {
+    HIPCHK(c, hipMemcpyAsync(il, c->psi, c->N * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(ev, c->stream));
+    HIPCHK(c, hipEventSynchronize(ev));
+  } else {
+    std::vector<float> f(2 * c->N);
+    HIPCHK(c, hipMemcpyAsync(f.data(), c->psi, c->N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(ev, c->stream));
+    HIPCHK(c, hipEventSynchronize(ev));
+    for (long i = 0; i < 2 * c->N; ++i) il[i] = f[i];
+  }
+  return GPUE_OK;
+}
+
+extern "C" {
+
 gpue_status gpue_parsum(gpue_ctx* c) {
   if (!c) return GPUE_EINVAL;
   gpue_status st = apply_pending(c);
@@ -743,11 +946,33 @@ gpue_status gpue_parsum(gpue_ctx* c) {
 // Queue the energy terms (kinetic by Parseval, position terms, GPUE's
 // forward-difference gauge term) into results[1..5]; copy results[0..5] to
 // host_dst (pinned memory for an asynchronous caller) when given.
+// The fields the energy kernels read: time-dependent ones are filled into their
+// tables at the time of the last step (one kernel per energy evaluation).
+static gpue_status energy_phys(gpue_ctx* c, Phys* out, bool fill) {
+  Phys ph = make_phys(c, 0, c->t_last);
+  if (c->t_dep) {
+    if (fill) {
+      hipError_t e = launch_fields_at(ph, (ph.v_kind == V_EQN || ph.v_kind == V_TRAP_A) ? c->Vt : nullptr,
+                                      c->deq[1] ? c->Kt : nullptr, c->deq[2] ? c->At[0] : nullptr,
+                                      c->deq[3] ? c->At[1] : nullptr, c->deq[4] ? c->At[2] : nullptr, c->N, c->stream);
+      if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("field tables: ") + hipGetErrorString(e));
+    }
+    if (ph.v_kind == V_EQN || ph.v_kind == V_TRAP_A) ph.v_kind = V_TABLE;
+    if (ph.k_kind == K_EQN) ph.k_kind = K_TABLE;
+    if (ph.a_kind == A_EQN) ph.a_kind = A_TABLE;
+    ph.eqk = 0;
+  }
+  *out = ph;
+  return GPUE_OK;
+}
+
 gpue_status gpue::energy_launch(gpue_ctx* c, double* host_dst) {
   gpue_status st = apply_pending(c);
   if (st != GPUE_OK) return st;
   const gpue_params& p = c->p;
-  Phys ph = make_phys(c, 0, 0.0);
+  Phys ph;
+  st = energy_phys(c, &ph, true);
+  if (st != GPUE_OK) return st;
   const double invN = 1.0 / (double)c->N;
   // kinetic term by Parseval over the full transform
   if (p.dimnum == 1) {
@@ -811,8 +1036,10 @@ gpue_status gpue_energy_consistent(gpue_ctx* c, double* energy, double* mu) {
   const gpue_params& p = c->p;
   const bool gauge = p.corotating && p.dimnum > 1 && c->a_kind != A_ZERO;
   // Az of the rotation gauge is 0; a table Az only when set and not all zero
-  const int naxes = !gauge ? 0 : (p.dimnum == 3 && c->a_kind == A_TABLE && !c->az_zero) ? 3 : 2;
-  const Phys ph = make_phys(c, 0, 0.0);
+  const int naxes = !gauge ? 0 : (p.dimnum == 3 && (c->a_kind == A_TABLE || c->a_kind == A_EQN) && !c->az_zero) ? 3 : 2;
+  Phys ph;
+  st = energy_phys(c, &ph, false);   // the tables energy_launch filled
+  if (st != GPUE_OK) return st;
   for (int a = 0; a < naxes && st == GPUE_OK; ++a)
     st = launch(c, {a == 0, a, a == 0 ? (int)RM_KSUM : (int)CM_KSUM, F_ASUM, 1.0 / (double)c->dims[a]}, ph, c->psi,
                 c->psi);
@@ -960,7 +1187,7 @@ static gpue_status step_cost(gpue_ctx* c, const int* v, hipEvent_t e0, hipEvent_
 // graph-replayed steps on a copy of the state (psi, the pending norm and the
 // step counter are restored).  A change is kept when it saves over 0.5 %.
 static gpue_status refine_steps(gpue_ctx* c, hipEvent_t e0, hipEvent_t e1) {
-  if (c->graph_steps <= 0 || c->v_time || c->ramp_type) return GPUE_OK;
+  if (c->graph_steps <= 0 || c->t_dep || c->ramp_type) return GPUE_OK;
   const int next_step = c->next_step;
   HIPCHK(c, hipMemcpyAsync(c->tmp, c->psi, c->N * c->esz, hipMemcpyDeviceToDevice, c->stream));
   std::swap(c->psi, c->tmp);
@@ -1045,22 +1272,15 @@ static gpue_status autotune(gpue_ctx* c, int reps) {
   return GPUE_OK;
 }
 
-// A time-dependent V is evaluated at time Dt*i over the grid before step i
-// (both half steps of GPUE's step use that time, src/evolution.cu:732,785).
-static gpue_status refresh_V(gpue_ctx* c, double t) {
-  const gpue_params& p = c->p;
-  hipError_t e = launch_eqn_field(c->veq, c->Vt, c->dims[0], c->dims[1], c->dims[2], c->d[0], c->d[1], c->d[2], p.xMax,
-                                  p.dimnum > 1 ? p.yMax : 0.0, p.dimnum > 2 ? p.zMax : 0.0, t, c->stream);
-  if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
-  return GPUE_OK;
-}
-
 }  // extern "C"
 
-// Sweeps of steps first..first+n-1 with, in real time, every row sweep that
-// ends a step fused with the RM_FWD that starts the next (no norm stands
-// between them, and the fields do not change): the pair's arithmetic is
-// unchanged, one read and one write of psi are saved.
+// Sweeps of steps first..first+n-1 with, in real time, the sweep that ends a
+// step fused with the one that starts the next when both run along the same
+// axis (no norm stands between them, and the fields do not change): the pair's
+// arithmetic is unchanged, one read and one write of psi are saved.  With a
+// gauge this holds at every step boundary (GPUE's gauge order alternates, so
+// a step ends on the axis the next one leads with): 2D real-time steps take 3
+// sweeps.
 void gpue::schedule_block(const gpue_ctx* c, int gstate, int first, int n, std::vector<Sweep>& out,
                           std::vector<int>& step_of) {
   std::vector<Sweep> one;
@@ -1073,20 +1293,26 @@ void gpue::schedule_block(const gpue_ctx* c, int gstate, int first, int n, std::
       step_of.push_back(first + s);
     }
   }
-  if (gstate != 1 || c->v_time || c->ramp_type) return;
+  if (gstate != 1 || c->t_dep || c->ramp_type) return;
   std::vector<Sweep> fused;
   std::vector<int> fstep;
   for (size_t k = 0; k < out.size(); ++k) {
     const Sweep& a = out[k];
-    if (k + 1 < out.size() && step_of[k + 1] == step_of[k] + 1 && a.row && (a.mode == RM_INV || a.mode == RM_GAUGE) &&
-        out[k + 1].row && out[k + 1].mode == RM_FWD && !(out[k + 1].flags & F_SCALE_IN)) {
-      Sweep m = a;
-      m.mode = a.mode == RM_INV ? RM_INV_FWD : RM_GAUGE_FWD;
-      m.flags |= out[k + 1].flags & F_HALFV_PRE;
-      fused.push_back(m);
-      fstep.push_back(step_of[k]);
-      ++k;
-      continue;
+    if (k + 1 < out.size() && step_of[k + 1] == step_of[k] + 1 && opens_step(out[k + 1]) &&
+        out[k + 1].row == a.row && out[k + 1].axis == a.axis && !(out[k + 1].flags & F_SCALE_IN)) {
+      int mode = -1;
+      if (a.row && a.mode == RM_INV) mode = RM_INV_FWD;
+      if (a.row && a.mode == RM_GAUGE) mode = RM_GAUGE_FWD;
+      if (!a.row && a.mode == CM_GAUGE) mode = CM_GAUGE_FWD;
+      if (mode >= 0) {
+        Sweep m = a;
+        m.mode = mode;
+        m.flags |= out[k + 1].flags & F_HALFV_PRE;
+        fused.push_back(m);
+        fstep.push_back(step_of[k]);
+        ++k;
+        continue;
+      }
     }
     fused.push_back(a);
     fstep.push_back(step_of[k]);
@@ -1130,7 +1356,8 @@ static gpue_status build_graph(gpue_ctx* c, int gstate, int len, StepGraph** out
     size_t lru = 0;
     for (size_t k = 1; k < c->graphs.size(); ++k)
       if (c->graphs[k].last_use < c->graphs[lru].last_use) lru = k;
-    HIPCHK(c, hipStreamSynchronize(c->stream));   // its last replay may still be queued
+    // its last replay may still be queued: wait for the latest replay of any block
+    if (c->gev) HIPCHK(c, hipEventSynchronize(c->gev));
     (void)hipGraphExecDestroy(c->graphs[lru].exec);
     (void)hipGraphDestroy(c->graphs[lru].graph);
     c->graphs.erase(c->graphs.begin() + lru);
@@ -1161,7 +1388,7 @@ static gpue_status step_graph(gpue_ctx* c, int gstate, int len, StepGraph** out)
 static int block_len(const gpue_ctx* c, int first_step, int nsteps, int s, int energy_every) {
   const int G = c->graph_steps;
   const int i = first_step + s;
-  if (!(G > 0 && !c->timing && !c->v_time && !c->ramp_type && (i % 2) == 0)) return 0;
+  if (!(G > 0 && !c->timing && !c->t_dep && !c->ramp_type && (i % 2) == 0)) return 0;
   int L = std::min(G, nsteps - s);
   if (energy_every > 0) {
     const int nxt = ((i + energy_every - 1) / energy_every) * energy_every;   // first cadence step >= i
@@ -1199,19 +1426,18 @@ static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gsta
       if (st != GPUE_OK) return st;
       sg->last_use = ++c->graph_clock;
       HIPCHK(c, hipGraphLaunch(sg->exec, c->stream));
+      if (!c->gev) HIPCHK(c, hipEventCreateWithFlags(&c->gev, hipEventDisableTiming));
+      HIPCHK(c, hipEventRecord(c->gev, c->stream));
       if (gstate == 0) c->pending_norm = true;
       s += L - 1;
       continue;
     }
     Phys ph = make_phys(c, gstate, Dt * i);   // time = Dt*i (src/evolution.cu:408)
     if (c->ramp_type) ph.Dt = Dt * ramp_scale(c, i);
-    if (c->v_time) {
-      gpue_status st = refresh_V(c, Dt * i);
-      if (st != GPUE_OK) return st;
-    }
+    c->t_last = Dt * i;
     schedule(c, gstate, i, c->pending_norm, sched);
     if (c->defer_nb > 0) {   // the previous step left its norm partials to this one
-      if (sched[0].row && sched[0].mode == RM_FWD && (sched[0].flags & F_SCALE_IN)) {
+      if (opens_step(sched[0]) && (sched[0].flags & F_SCALE_IN)) {
         sched[0].flags |= F_SCALE_PART;
         sched[0].part_nb = c->defer_nb;
         c->defer_nb = 0;
@@ -1466,10 +1692,20 @@ gpue_status gpue_math_eval(int device, int fn, const double* x, double* y, long 
 
 gpue_status gpue_step_bytes(gpue_ctx* c, int gstate, double* bytes, double* sweeps) {
   if (!c || (gstate != 0 && gstate != 1)) return GPUE_EINVAL;
-  std::vector<Sweep> a, b;
-  schedule(c, gstate, 0, false, a);
-  schedule(c, gstate, 1, false, b);
-  const double P = 0.5 * (double)(a.size() + b.size());
+  // the sweeps the steady state runs: a captured block when graphs are on (real
+  // time fuses the sweeps at step boundaries), else a step of each parity
+  const int G = c->graph_steps > 0 && !c->t_dep && !c->ramp_type ? c->graph_steps : 2;
+  std::vector<Sweep> blk;
+  std::vector<int> step_of;
+  if (G == c->graph_steps) {
+    schedule_block(c, gstate, 0, G, blk, step_of);
+  } else {
+    std::vector<Sweep> a, b;
+    schedule(c, gstate, 0, false, a);
+    schedule(c, gstate, 1, false, b);
+    blk.resize(a.size() + b.size());
+  }
+  const double P = (double)blk.size() / (double)G;
   if (sweeps) *sweeps = P;
   if (bytes) *bytes = 2.0 * (double)c->esz * (double)c->N * P;
   return GPUE_OK;

@@ -71,6 +71,7 @@ struct Slot {
   gpue_ctx* ctx = nullptr;
   double* host = nullptr;          // pinned: energy terms read back after a chunk
   hipEvent_t ev = nullptr;
+  int cls = 0;                     // size class: the stream the request runs on
   bool busy = false;
   int gstate = 0;
   int step = 0;                    // step index within the current phase
@@ -84,10 +85,38 @@ struct Slot {
 
 }  // namespace
 
+// A finished request's context, pinned buffer and event, kept for the next
+// request of the same shape (its device memory stays counted in the budget).
+struct Pooled {
+  gpue_ctx* ctx = nullptr;
+  double* host = nullptr;
+  hipEvent_t ev = nullptr;
+  double bytes = 0.0;
+  int cls = 0;
+  long last_use = 0;
+};
+
+// Requests run on at most NSTREAMS streams, one per size class, since the device
+// gives a process four hardware queues (more streams would share them in FIFO
+// order anyway): large states keep one queue to themselves and small ones are
+// never queued behind them.
+constexpr int NSTREAMS = 4;
+static int size_class(const gpue_params& p) {
+  const double n = (double)p.xDim * (p.dimnum > 1 ? p.yDim : 1) * (p.dimnum > 2 ? p.zDim : 1);
+  const double bytes = n * (p.precision == 32 ? 8.0 : 16.0);
+  if (bytes >= 64.0 * 1024 * 1024) return 0;   // 256^3-class states
+  if (bytes >= 8.0 * 1024 * 1024) return 1;    // 1024^2, 128^3-192^3
+  if (bytes >= 1.0 * 1024 * 1024) return 2;    // 256^2-512^2, 64^3
+  return 3;
+}
+
 struct gpue_server {
   int device = 0;
   double budget = 0.0, used = 0.0;
   int max_slots = 0;
+  hipStream_t streams[NSTREAMS] = {};
+  std::vector<Pooled> pool;
+  long pool_clock = 0;
   std::chrono::steady_clock::time_point t0;
   std::mutex mu;
   std::condition_variable cv;
@@ -122,16 +151,44 @@ static gpue_status issue_chunk(Slot* s, int nsteps) {
   return GPUE_OK;
 }
 
-static gpue_status admit(gpue_server* sv, const Req& q, Slot** out) {
+// Free pooled contexts, least recently used first, until `need` more bytes fit
+// the budget (hipFree waits for the whole device, so this happens only when the
+// budget demands it).
+static void shrink_pool(gpue_server* sv, double need) {
+  while (!sv->pool.empty() && sv->used + need > sv->budget) {
+    size_t k = 0;
+    for (size_t j = 1; j < sv->pool.size(); ++j)
+      if (sv->pool[j].last_use < sv->pool[k].last_use) k = j;
+    Pooled& e = sv->pool[k];
+    if (e.ev) (void)hipEventDestroy(e.ev);
+    if (e.host) (void)hipHostFree(e.host);
+    gpue_destroy(e.ctx);
+    sv->used -= e.bytes;
+    sv->pool.erase(sv->pool.begin() + k);
+  }
+}
+
+// Admit a request: `reuse` is a pooled context of its shape taken out of the
+// pool at admission (ctx null: none), else a new context is created.
+static gpue_status admit(gpue_server* sv, const Req& q, const Pooled& reuse, Slot** out) {
   Slot* s = new Slot();
   s->req = q;
   s->bytes = footprint(q.r.params);
   gpue_params p = q.r.params;
   p.device = sv->device;
-  gpue_status st = gpue_create(&p, &s->ctx);
-  if (st == GPUE_OK && (hipHostMalloc((void**)&s->host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-                        hipEventCreateWithFlags(&s->ev, hipEventDisableTiming) != hipSuccess))
-    st = GPUE_EHIP;
+  s->cls = size_class(p);
+  gpue_status st = GPUE_OK;
+  if (reuse.ctx) {   // no device allocation
+    s->ctx = reuse.ctx;
+    s->host = reuse.host;
+    s->ev = reuse.ev;
+    st = ctx_reset(s->ctx, &p);
+  } else {
+    st = ctx_create(&p, sv->streams[s->cls], &s->ctx);
+    if (st == GPUE_OK && (hipHostMalloc((void**)&s->host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+                          hipEventCreateWithFlags(&s->ev, hipEventDisableTiming) != hipSuccess))
+      st = GPUE_EHIP;
+  }
   if (st == GPUE_OK) {
     gpue_ctx* c = s->ctx;
     const long N = c->N;
@@ -166,7 +223,8 @@ static void retire(gpue_server* sv, Slot* s) {
   r.t_submit = s->req.t_submit;
   r.t_start = s->t_start;
   if (s->ctx && s->req.r.psi_out && s->status == GPUE_OK) {
-    const gpue_status st = gpue_download_wfc(s->ctx, s->req.r.psi_out);
+    // waits for this request's copy, not for the later work of the stream's other requests
+    const gpue_status st = download_wfc_ev(s->ctx, s->req.r.psi_out, s->ev);
     if (st != GPUE_OK) r.status = st;
   }
   r.t_done = sv->clock();
@@ -178,12 +236,24 @@ static void retire(gpue_server* sv, Slot* s) {
   sv->inflight--;
 }
 
+// Finished requests' contexts go to the pool (their memory stays counted).
 static void release_retired(gpue_server* sv) {
   for (Slot* s : sv->retired) {
-    if (s->ev) (void)hipEventDestroy(s->ev);
-    if (s->host) (void)hipHostFree(s->host);
-    if (s->ctx) gpue_destroy(s->ctx);
-    sv->used -= s->bytes;
+    if (s->ctx && s->status == GPUE_OK) {
+      Pooled e;
+      e.ctx = s->ctx;
+      e.host = s->host;
+      e.ev = s->ev;
+      e.bytes = s->bytes;
+      e.cls = s->cls;
+      e.last_use = ++sv->pool_clock;
+      sv->pool.push_back(e);
+    } else {   // a failed request's context is not reused
+      if (s->ev) (void)hipEventDestroy(s->ev);
+      if (s->host) (void)hipHostFree(s->host);
+      if (s->ctx) gpue_destroy(s->ctx);
+      sv->used -= s->bytes;
+    }
     delete s;
   }
   sv->retired.clear();
@@ -234,38 +304,49 @@ static void advance(gpue_server* sv, Slot* s) {
 static void worker_main(gpue_server* sv) {
   (void)hipSetDevice(sv->device);
   for (;;) {
-    // admissions, in arrival order, while a slot and the HBM budget allow;
-    // retired slots are released when nothing runs or the budget needs them
+    // admissions, in arrival order, while a slot and the HBM budget allow; a
+    // pooled context of the request's shape is reused, other pooled contexts
+    // are freed only when the budget needs their memory
     std::vector<Req> admitted;
-    bool release = false;
+    std::vector<Pooled> reuse;
+    double shrink = 0.0;
     {
       std::unique_lock<std::mutex> lk(sv->mu);
       if (sv->active.empty() && sv->queue.empty()) {
-        if (sv->retired.empty()) {
-          if (sv->stop) break;
-          sv->cv.wait_for(lk, std::chrono::milliseconds(5));
-          continue;
-        }
-        release = true;
+        if (sv->stop) break;
+        sv->cv.wait_for(lk, std::chrono::milliseconds(5));
+        continue;
       }
-      while (!release && !sv->queue.empty() && (int)(sv->active.size() + admitted.size()) < sv->max_slots) {
-        const double need = footprint(sv->queue.front().r.params);
-        if (sv->used + need > sv->budget) {
-          release = !sv->retired.empty();
+      while (!sv->queue.empty() && (int)(sv->active.size() + admitted.size()) < sv->max_slots) {
+        gpue_params qp = sv->queue.front().r.params;
+        qp.device = sv->device;
+        Pooled e;
+        for (size_t k = 0; k < sv->pool.size(); ++k)
+          if (ctx_same_shape(sv->pool[k].ctx, &qp)) {
+            e = sv->pool[k];
+            sv->pool.erase(sv->pool.begin() + k);
+            break;
+          }
+        const double extra = e.ctx ? 0.0 : footprint(qp);   // a pooled context is already counted
+        if (sv->used + extra > sv->budget) {
+          if (e.ctx) sv->pool.push_back(e);
+          shrink = extra;
           break;
         }
-        sv->used += need;
+        sv->used += extra;
         admitted.push_back(sv->queue.front());
+        reuse.push_back(e);
         sv->queue.pop_front();
       }
     }
-    if (release) {
-      release_retired(sv);
-      continue;
+    if (shrink > 0.0 && admitted.empty()) {
+      const size_t before = sv->pool.size();
+      shrink_pool(sv, shrink);
+      if (sv->pool.size() != before) continue;   // room made: admit again
     }
-    for (const Req& q : admitted) {
+    for (size_t k = 0; k < admitted.size(); ++k) {
       Slot* s = nullptr;
-      admit(sv, q, &s);
+      admit(sv, admitted[k], reuse[k], &s);
       const gpue_request& r = s->req.r;
       const int every = r.check_every > 0 ? r.check_every : 100;
       if (s->status == GPUE_OK) {
@@ -306,6 +387,7 @@ static void worker_main(gpue_server* sv) {
         ++k;
       }
     }
+    release_retired(sv);
     if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
   release_retired(sv);
@@ -320,6 +402,16 @@ gpue_status gpue_server_create(int device, double budget, int max_slots, gpue_se
   sv->budget = budget;
   sv->max_slots = max_slots;
   sv->t0 = std::chrono::steady_clock::now();
+  if (hipSetDevice(device) != hipSuccess) {
+    delete sv;
+    return GPUE_EHIP;
+  }
+  for (int k = 0; k < NSTREAMS; ++k)
+    if (hipStreamCreateWithFlags(&sv->streams[k], hipStreamNonBlocking) != hipSuccess) {
+      for (int j = 0; j < k; ++j) (void)hipStreamDestroy(sv->streams[j]);
+      delete sv;
+      return GPUE_EHIP;
+    }
   sv->worker = std::thread(worker_main, sv);
   *out = sv;
   return GPUE_OK;
@@ -367,6 +459,9 @@ void gpue_server_destroy(gpue_server* sv) {
   sv->stop = true;
   sv->cv.notify_all();
   if (sv->worker.joinable()) sv->worker.join();
+  shrink_pool(sv, 2.0 * sv->budget + 1.0);   // every pooled context
+  for (int k = 0; k < NSTREAMS; ++k)
+    if (sv->streams[k]) (void)hipStreamDestroy(sv->streams[k]);
   delete sv;
 }
 

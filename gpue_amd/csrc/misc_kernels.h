@@ -9,6 +9,8 @@ constexpr int REDUCE_BLOCKS = 1024;   // grid of the standalone reductions
 struct EqnProg;
 hipError_t launch_eqn_field(const EqnProg& prog, double* out, int xDim, int yDim, int zDim, double dx, double dy,
                             double dz, double xMax, double yMax, double zMax, double t, hipStream_t s);
+hipError_t launch_fields_at(const Phys& p, double* V, double* K, double* A0, double* A1, double* A2, long N,
+                            hipStream_t s);
 template <typename T>
 hipError_t launch_apply_norm(void* psi, long N, const double* norm, hipStream_t s);
 template <typename T>

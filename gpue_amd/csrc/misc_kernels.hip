@@ -55,6 +55,19 @@ __global__ void k_eqn_field(EqnProg prog, double* out, int xDim, int yDim, int z
   }
 }
 
+// Time-dependent fields at time p.t into their tables (for the energy): the same
+// per-point values the EQ sweeps evaluate.
+__global__ void k_fields_at(Phys p, double* V, double* K, double* A0, double* A1, double* A2, long N) {
+  for (long gid = blockIdx.x * (long)blockDim.x + threadIdx.x; gid < N; gid += (long)gridDim.x * blockDim.x) {
+    const Pt q{(int)(gid % p.xDim), (int)((gid / p.xDim) % p.yDim), (int)(gid / ((long)p.xDim * p.yDim))};
+    if (V) V[gid] = V_eq(p, q, gid);
+    if (K) K[gid] = eq_field(p, p.eq[1], q);
+    if (A0) A0[gid] = eq_field(p, p.eq[2], q);
+    if (A1) A1[gid] = eq_field(p, p.eq[3], q);
+    if (A2) A2[gid] = eq_field(p, p.eq[4], q);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // psi /= norm  (pending renormalisation made explicit)
 // ---------------------------------------------------------------------------
@@ -285,6 +298,13 @@ static inline int gs_blocks(long n, int bs, int cap) {
   if (b > cap) b = cap;
   if (b < 1) b = 1;
   return (int)b;
+}
+
+hipError_t launch_fields_at(const Phys& p, double* V, double* K, double* A0, double* A1, double* A2, long N,
+                            hipStream_t s) {
+  const int blocks = (int)std::min<long>((N + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fields_at, dim3(blocks), dim3(256), 0, s, p, V, K, A0, A1, A2, N);
+  return hipGetLastError();
 }
 
 hipError_t launch_eqn_field(const EqnProg& prog, double* out, int xDim, int yDim, int zDim, double dx, double dy,

@@ -60,9 +60,11 @@ __device__ __forceinline__ double abs2d(cx<T> v) {
 // ---------------------------------------------------------------------------
 template <typename T, int N, int RV, int MODE_>
 __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) {
-  // bit 16 of the mode: FP32 real-time kernels forming phase factors in FP64
+  // bit 16 of the mode: FP32 real-time kernels forming phase factors in FP64;
+  // bit 32: time-dependent equation fields evaluated per point (EQ kernels)
   constexpr int MODE = MODE_ & 15;
   constexpr bool X = (MODE_ & 16) != 0;
+  constexpr bool EQ = (MODE_ & 32) != 0;
   constexpr int V = RV & 1;
   constexpr bool PRE = (RV >> 1) != 0;
   using P = Plan<N, V>;
@@ -95,10 +97,10 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
     }
     if constexpr (PRE) twr.load(tid, tw);
     fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
-    if (flags & F_HALFV_POST) halfV_all<T, E, X>(p, v, pos0);
+    if (flags & F_HALFV_POST) halfV_all<T, E, X, EQ>(p, v, pos0);
     if (flags & F_GAUGE_POST) {
       fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
-      gauge_all<T, E, 0, X>(p, v, posL, a.invN);
+      gauge_all<T, E, 0, X, EQ>(p, v, posL, a.invN);
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     }
   } else {
@@ -123,19 +125,22 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 #pragma unroll
         for (int r = 0; r < E; ++r) v[r] = scale_norm(v[r], inv);
       }
-      if (flags & F_HALFV_PRE) halfV_all<T, E, X>(p, v, pos0);
+      if (flags & F_HALFV_PRE) halfV_all<T, E, X, EQ>(p, v, pos0);
     }
     fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
     if constexpr (MODE == RM_GAUGE || MODE == RM_GAUGE_FWD) {
-      gauge_all<T, E, 0, X>(p, v, posL, a.invN);
+      gauge_all<T, E, 0, X, EQ>(p, v, posL, a.invN);
+      fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
+    } else if constexpr (MODE == RM_K) {
+      kstep_all<T, E, X, EQ>(p, v, posL, a.invN);
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     } else if constexpr (MODE == RM_FULL) {
-      kstep_all<T, E, X>(p, v, posL, a.invN);
+      kstep_all<T, E, X, EQ>(p, v, posL, a.invN);
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
-      if (flags & F_HALFV_POST) halfV_all<T, E, X>(p, v, pos0);
+      if (flags & F_HALFV_POST) halfV_all<T, E, X, EQ>(p, v, pos0);
       if (flags & F_GAUGE_POST) {
         fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
-        gauge_all<T, E, 0, X>(p, v, posL, a.invN);
+        gauge_all<T, E, 0, X, EQ>(p, v, posL, a.invN);
         fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
       }
     } else if constexpr (MODE == RM_KSUM) {
@@ -153,7 +158,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 
   // bridge modes continue with the next step's first row sweep
   if constexpr (MODE == RM_INV_FWD || MODE == RM_GAUGE_FWD) {
-    if (flags & F_HALFV_PRE) halfV_all<T, E, X>(p, v, pos0);
+    if (flags & F_HALFV_PRE) halfV_all<T, E, X, EQ>(p, v, pos0);
     fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
   }
 
@@ -184,9 +189,11 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
 // ---------------------------------------------------------------------------
 template <typename T, int N, int CV, int MODE_>
 __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) {
-  // bit 16 of the mode: FP32 real-time kernels forming phase factors in FP64
+  // bit 16 of the mode: FP32 real-time kernels forming phase factors in FP64;
+  // bit 32: time-dependent equation fields evaluated per point (EQ kernels)
   constexpr int MODE = MODE_ & 15;
   constexpr bool X = (MODE_ & 16) != 0;
+  constexpr bool EQ = (MODE_ & 32) != 0;
   constexpr int V = CV & 1;
   constexpr bool PRE = (CV >> 1) != 0;
   using P = Plan<N, V>;
@@ -216,17 +223,43 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
   auto yz = [&](int e, int& yi, int& zi) {
     if (a.axis == 1) { yi = e; zi = o; } else { yi = o; zi = e; }
   };
+  // grid indices of register slot r in the boundary layouts of the first / last radix
+  auto pos0 = [&](int r) {
+    const int e = bpos<N, TPL, RF0>(tid, r);
+    return a.axis == 1 ? Pt{xq, e, o} : Pt{xq, o, e};
+  };
+  auto posL = [&](int r) {
+    const int e = bpos<N, TPL, RFL>(tid, r);
+    return a.axis == 1 ? Pt{xq, e, o} : Pt{xq, o, e};
+  };
+  auto gauge_here = [&](cx<T>* w) {
+    if (a.axis == 1) gauge_all<T, E, 1, X, EQ>(p, w, posL, a.invN);
+    else gauge_all<T, E, 2, X, EQ>(p, w, posL, a.invN);
+  };
 
-  if constexpr (MODE == CM_INV) {
+  if constexpr (MODE == CM_INV || MODE == CM_INV_VG) {
     if (active) {
 #pragma unroll
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(tid, r) * es];
     }
     if constexpr (PRE) twr.load(tid, tw);
     fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
+    if constexpr (MODE == CM_INV_VG) {
+      // the closing sweep of a step that leads with this axis: V/2, then this
+      // axis' gauge operator
+      if (flags & F_HALFV_POST) halfV_all<T, E, X, EQ>(p, v, pos0);
+      if (flags & F_GAUGE_POST) {
+        fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
+        gauge_here(v);
+        fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
+      }
+    }
     if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) out[bpos<N, TPL, RF0>(tid, r) * es] = v[r];
+      for (int r = 0; r < E; ++r) {
+        out[bpos<N, TPL, RF0>(tid, r) * es] = v[r];
+        if (MODE == CM_INV_VG && (flags & F_NORM)) acc += abs2d(v[r]);
+      }
     }
   } else {
     if (active) {
@@ -234,13 +267,27 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
       for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RF0>(tid, r) * es];
     }
     if constexpr (PRE) twr.load(tid, tw);
-    auto posL = [&](int r) {
-      const int e = bpos<N, TPL, RFL>(tid, r);
-      return a.axis == 1 ? Pt{xq, e, o} : Pt{xq, o, e};
-    };
+    if constexpr (MODE == CM_FWD_V) {
+      // the opening sweep of a step that leads with this axis
+      if (flags & F_SCALE_IN) {
+        // a deferred renormalisation (as in the row sweep): the norm reduced
+        // here from the previous sweep's partials by every wave, stored once
+        double nrm;
+        if (flags & F_SCALE_PART) {
+          nrm = part_norm(a.partials, a.norm_nb, a.norm_scale);
+          if (bid == 0 && threadIdx.x == 0) *a.norm_out = nrm;
+        } else {
+          nrm = *a.norm_in;
+        }
+        const T inv = (T)(1.0 / nrm);
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = scale_norm(v[r], inv);
+      }
+      if (flags & F_HALFV_PRE) halfV_all<T, E, X, EQ>(p, v, pos0);
+    }
     auto epos = [&](int r) { return bpos<N, TPL, RFL>(tid, r); };
     fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
-    if constexpr (MODE == CM_FWD) {
+    if constexpr (MODE == CM_FWD || MODE == CM_FWD_V) {
       if (active) {
 #pragma unroll
         for (int r = 0; r < E; ++r) out[bpos<N, TPL, RFL>(tid, r) * es] = v[r];
@@ -260,18 +307,27 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
       }
     } else {
       if constexpr (MODE == CM_K) {
-        if (a.axis == 1) kstep_col<T, E, 1, X>(p, v, xq, o, epos, a.invN);
-        else kstep_col<T, E, 2, X>(p, v, xq, o, epos, a.invN);
+        if (a.axis == 1) kstep_col<T, E, 1, X, EQ>(p, v, xq, o, epos, a.invN);
+        else kstep_col<T, E, 2, X, EQ>(p, v, xq, o, epos, a.invN);
       } else {
-        if (a.axis == 1) gauge_all<T, E, 1, X>(p, v, posL, a.invN);
-        else gauge_all<T, E, 2, X>(p, v, posL, a.invN);
+        gauge_here(v);
       }
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
-      if (active) {
+      if constexpr (MODE == CM_GAUGE_FWD) {
+        // bridge: the next step's V/2 and its first transform along this axis
+        if (flags & F_HALFV_PRE) halfV_all<T, E, X, EQ>(p, v, pos0);
+        fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
+        if (active) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) {
-          out[bpos<N, TPL, RF0>(tid, r) * es] = v[r];
-          if (flags & F_NORM) acc += abs2d(v[r]);
+          for (int r = 0; r < E; ++r) out[bpos<N, TPL, RFL>(tid, r) * es] = v[r];
+        }
+      } else {
+        if (active) {
+#pragma unroll
+          for (int r = 0; r < E; ++r) {
+            out[bpos<N, TPL, RF0>(tid, r) * es] = v[r];
+            if (flags & F_NORM) acc += abs2d(v[r]);
+          }
         }
       }
     }

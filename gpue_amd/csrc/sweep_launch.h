@@ -13,8 +13,65 @@ inline int col_blocks(int xDim, int others) {
 
 // FP32 real-time steps run the kernels whose mode carries bit 16 (phase factors
 // and their products in FP64, sweep_kernels.h); every other launch the plain ones.
+// Kernels with time-dependent equation fields evaluated per point (bit 32 of
+// the mode): only for the default variants (wide lines, or narrow lines with
+// quarter strided tiles), which contexts with such fields always use; no
+// bridge modes (their steps run one by one).
+// FP64 only: GPUE's fields are FP64, and FP32 contexts reject time-dependent ones.
+template <typename T, int RV>
+constexpr bool row_eq_variant() { return sizeof(T) == 8 && (RV == 0 || RV == 3); }
+template <typename T, int CV>
+constexpr bool col_eq_variant() { return sizeof(T) == 8 && (CV == 0 || CV == 5); }
+
+template <typename T, int N, int V>
+hipError_t launch_row_eq(int mode, const SweepArgs& a, hipStream_t s) {
+  if constexpr (row_eq_variant<T, V>()) {
+    dim3 grid(row_blocks<N, V>(a.lines)), block(RowCfg<N, V>::WG);
+#define GPUE_ROW_EQ(M_)                                                                          \
+  hipLaunchKernelGGL((row_sweep<T, N, V, M_ | 32>), grid, block, 0, s, a);                       \
+  break;
+    switch (mode) {
+      case RM_FWD: GPUE_ROW_EQ(RM_FWD)
+      case RM_INV: GPUE_ROW_EQ(RM_INV)
+      case RM_GAUGE: GPUE_ROW_EQ(RM_GAUGE)
+      case RM_FULL: GPUE_ROW_EQ(RM_FULL)
+      case RM_K: GPUE_ROW_EQ(RM_K)
+      default: return hipErrorInvalidValue;
+    }
+#undef GPUE_ROW_EQ
+    return hipGetLastError();
+  } else {
+    (void)mode; (void)a; (void)s;
+    return hipErrorInvalidValue;
+  }
+}
+
+template <typename T, int N, int V>
+hipError_t launch_col_eq(int mode, const SweepArgs& a, hipStream_t s) {
+  if constexpr (col_eq_variant<T, V>()) {
+    const int others = a.axis == 1 ? a.ph.zDim : a.ph.yDim;
+    dim3 grid(col_blocks<T, N, V>(a.ph.xDim, others)), block(ColCfg<T, N, V>::WG);
+#define GPUE_COL_EQ(M_)                                                                          \
+  hipLaunchKernelGGL((col_sweep<T, N, V, M_ | 32>), grid, block, 0, s, a);                       \
+  break;
+    switch (mode) {
+      case CM_K: GPUE_COL_EQ(CM_K)
+      case CM_GAUGE: GPUE_COL_EQ(CM_GAUGE)
+      case CM_FWD_V: GPUE_COL_EQ(CM_FWD_V)
+      case CM_INV_VG: GPUE_COL_EQ(CM_INV_VG)
+      default: return hipErrorInvalidValue;
+    }
+#undef GPUE_COL_EQ
+    return hipGetLastError();
+  } else {
+    (void)mode; (void)a; (void)s;
+    return hipErrorInvalidValue;
+  }
+}
+
 template <typename T, int N, int V>
 hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
+  if (a.ph.eqk && mode != RM_KSUM) return launch_row_eq<T, N, V>(mode, a, s);
   dim3 grid(row_blocks<N, V>(a.lines)), block(RowCfg<N, V>::WG);
   if constexpr (sizeof(T) == 4) {
     if (a.ph.gstate != 0 && mode != RM_KSUM) {
@@ -25,6 +82,7 @@ hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
         case RM_FULL: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FULL | 16>), grid, block, 0, s, a); break;
         case RM_INV_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD | 16>), grid, block, 0, s, a); break;
         case RM_GAUGE_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE_FWD | 16>), grid, block, 0, s, a); break;
+        case RM_K: hipLaunchKernelGGL((row_sweep<T, N, V, RM_K | 16>), grid, block, 0, s, a); break;
         default: return hipErrorInvalidValue;
       }
       return hipGetLastError();
@@ -38,6 +96,7 @@ hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
     case RM_KSUM: hipLaunchKernelGGL((row_sweep<T, N, V, RM_KSUM>), grid, block, 0, s, a); break;
     case RM_INV_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD>), grid, block, 0, s, a); break;
     case RM_GAUGE_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE_FWD>), grid, block, 0, s, a); break;
+    case RM_K: hipLaunchKernelGGL((row_sweep<T, N, V, RM_K>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -45,12 +104,20 @@ hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
 
 template <typename T, int N, int V>
 hipError_t launch_col_t(int mode, const SweepArgs& a, hipStream_t s) {
+  if (a.ph.eqk && (mode == CM_K || mode == CM_GAUGE || mode == CM_FWD_V || mode == CM_INV_VG))
+    return launch_col_eq<T, N, V>(mode, a, s);
   const int others = a.axis == 1 ? a.ph.zDim : a.ph.yDim;
   dim3 grid(col_blocks<T, N, V>(a.ph.xDim, others)), block(ColCfg<T, N, V>::WG);
   if constexpr (sizeof(T) == 4) {
-    if (a.ph.gstate != 0 && (mode == CM_K || mode == CM_GAUGE)) {
-      if (mode == CM_K) hipLaunchKernelGGL((col_sweep<T, N, V, CM_K | 16>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((col_sweep<T, N, V, CM_GAUGE | 16>), grid, block, 0, s, a);
+    if (a.ph.gstate != 0 && col_phase_mode(mode)) {
+      switch (mode) {
+        case CM_K: hipLaunchKernelGGL((col_sweep<T, N, V, CM_K | 16>), grid, block, 0, s, a); break;
+        case CM_GAUGE: hipLaunchKernelGGL((col_sweep<T, N, V, CM_GAUGE | 16>), grid, block, 0, s, a); break;
+        case CM_FWD_V: hipLaunchKernelGGL((col_sweep<T, N, V, CM_FWD_V | 16>), grid, block, 0, s, a); break;
+        case CM_INV_VG: hipLaunchKernelGGL((col_sweep<T, N, V, CM_INV_VG | 16>), grid, block, 0, s, a); break;
+        case CM_GAUGE_FWD: hipLaunchKernelGGL((col_sweep<T, N, V, CM_GAUGE_FWD | 16>), grid, block, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+      }
       return hipGetLastError();
     }
   }
@@ -60,6 +127,9 @@ hipError_t launch_col_t(int mode, const SweepArgs& a, hipStream_t s) {
     case CM_K: hipLaunchKernelGGL((col_sweep<T, N, V, CM_K>), grid, block, 0, s, a); break;
     case CM_GAUGE: hipLaunchKernelGGL((col_sweep<T, N, V, CM_GAUGE>), grid, block, 0, s, a); break;
     case CM_KSUM: hipLaunchKernelGGL((col_sweep<T, N, V, CM_KSUM>), grid, block, 0, s, a); break;
+    case CM_FWD_V: hipLaunchKernelGGL((col_sweep<T, N, V, CM_FWD_V>), grid, block, 0, s, a); break;
+    case CM_INV_VG: hipLaunchKernelGGL((col_sweep<T, N, V, CM_INV_VG>), grid, block, 0, s, a); break;
+    case CM_GAUGE_FWD: hipLaunchKernelGGL((col_sweep<T, N, V, CM_GAUGE_FWD>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -77,6 +147,7 @@ hipError_t launch_row_b_t(int mode, const SweepArgs* d, int ninst, int lines, hi
         case RM_INV: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_INV | 16>), grid, block, 0, s, d, bpi); break;
         case RM_GAUGE: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_GAUGE | 16>), grid, block, 0, s, d, bpi); break;
         case RM_FULL: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_FULL | 16>), grid, block, 0, s, d, bpi); break;
+        case RM_K: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_K | 16>), grid, block, 0, s, d, bpi); break;
         default: return hipErrorInvalidValue;
       }
       return hipGetLastError();
@@ -87,6 +158,7 @@ hipError_t launch_row_b_t(int mode, const SweepArgs* d, int ninst, int lines, hi
     case RM_INV: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_INV>), grid, block, 0, s, d, bpi); break;
     case RM_GAUGE: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_GAUGE>), grid, block, 0, s, d, bpi); break;
     case RM_FULL: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_FULL>), grid, block, 0, s, d, bpi); break;
+    case RM_K: hipLaunchKernelGGL((row_sweep_b<T, N, V, RM_K>), grid, block, 0, s, d, bpi); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -101,6 +173,8 @@ hipError_t launch_col_b_t(int mode, const SweepArgs* d, int ninst, int xDim, int
       switch (mode & 15) {
         case CM_K: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_K | 16>), grid, block, 0, s, d, bpi); break;
         case CM_GAUGE: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_GAUGE | 16>), grid, block, 0, s, d, bpi); break;
+        case CM_FWD_V: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_FWD_V | 16>), grid, block, 0, s, d, bpi); break;
+        case CM_INV_VG: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_INV_VG | 16>), grid, block, 0, s, d, bpi); break;
         default: return hipErrorInvalidValue;
       }
       return hipGetLastError();
@@ -111,6 +185,8 @@ hipError_t launch_col_b_t(int mode, const SweepArgs* d, int ninst, int xDim, int
     case CM_INV: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_INV>), grid, block, 0, s, d, bpi); break;
     case CM_K: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_K>), grid, block, 0, s, d, bpi); break;
     case CM_GAUGE: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_GAUGE>), grid, block, 0, s, d, bpi); break;
+    case CM_FWD_V: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_FWD_V>), grid, block, 0, s, d, bpi); break;
+    case CM_INV_VG: hipLaunchKernelGGL((col_sweep_b<T, N, V, CM_INV_VG>), grid, block, 0, s, d, bpi); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

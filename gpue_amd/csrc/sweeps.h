@@ -19,16 +19,29 @@
 namespace gpue {
 
 // V: separable 1-D value tables (harmonic trap + 1/2 m |A|^2 of the rotation gauge)
-// or a full table (file / equation fields, V re-evaluated per step when time dependent).
-enum { V_SEP = 0, V_TABLE = 1 };
-enum { A_ROTATION = 0, A_ZERO = 1, A_TABLE = 2 };
-enum { K_SEP = 0, K_TABLE = 1 };
+// or a full table (file / static equation fields).  Time-dependent equation
+// fields are evaluated per point inside the step's sweeps (the EQ kernels):
+// V_EQN, K_EQN and A_EQN; V_TRAP_A is the harmonic trap plus 1/2 m |A|^2 of a
+// time-dependent A (GPUE builds V after A, src/operators.cu:726-829).
+enum { V_SEP = 0, V_TABLE = 1, V_EQN = 2, V_TRAP_A = 3 };
+enum { A_ROTATION = 0, A_ZERO = 1, A_TABLE = 2, A_EQN = 3 };
+enum { K_SEP = 0, K_TABLE = 1, K_EQN = 2 };
 
 // Sweep modes: row (x) and strided (y/z) kernels.
-// RM_INV_FWD / RM_GAUGE_FWD: the last row sweep of a real-time step fused with
-// the first (RM_FWD) of the next one: ..., [V/2 of step i+1 (F_HALFV_PRE)], FFT.
-enum { RM_FWD = 0, RM_INV = 1, RM_GAUGE = 2, RM_FULL = 3, RM_KSUM = 4, RM_INV_FWD = 5, RM_GAUGE_FWD = 6 };
-enum { CM_FWD = 0, CM_INV = 1, CM_K = 2, CM_GAUGE = 3, CM_KSUM = 4 };
+// RM_INV_FWD / RM_GAUGE_FWD / CM_GAUGE_FWD: the last sweep of a real-time step
+// fused with the first of the next one on the same axis:
+// ..., [V/2 of step i+1 (F_HALFV_PRE)], FFT.
+// RM_K: FFT, K, IFFT along x (a step that leads with the y axis).
+// CM_FWD_V: [scale] [V/2] FFT along y/z (a step that leads with that axis);
+// CM_INV_VG: IFFT [V/2] [FFT, A, IFFT] along y/z (its closing sweep).
+enum { RM_FWD = 0, RM_INV = 1, RM_GAUGE = 2, RM_FULL = 3, RM_KSUM = 4, RM_INV_FWD = 5, RM_GAUGE_FWD = 6, RM_K = 7 };
+enum { CM_FWD = 0, CM_INV = 1, CM_K = 2, CM_GAUGE = 3, CM_KSUM = 4, CM_FWD_V = 5, CM_INV_VG = 6, CM_GAUGE_FWD = 7 };
+
+// Strided-sweep modes that apply a phase operator (FP32 real time runs their
+// bit-16 kernels).
+__host__ __device__ inline bool col_phase_mode(int mode) {
+  return mode == CM_K || mode == CM_GAUGE || mode == CM_FWD_V || mode == CM_INV_VG || mode == CM_GAUGE_FWD;
+}
 
 // Flags of a sweep (runtime-uniform).
 enum : unsigned {
@@ -52,6 +65,7 @@ enum : int { EQ_CONST = 0, EQ_X = 1, EQ_Y = 2, EQ_Z = 3, EQ_T = 4, EQ_OP = 16 };
 constexpr int EQ_MAX = 48;
 struct EqnProg {
   int n;
+  int depth;   // largest stack depth the program reaches
   int op[EQ_MAX];
   double val[EQ_MAX];
 };
@@ -75,6 +89,13 @@ struct Phys {
   const float *vsf[3];   // FP32 contexts: vs and kf rounded to FP32 once (the FP32
   const float *kff[3];   // sweeps read these; null for FP64 contexts)
   const double *Vt, *Kt, *At[3];
+  // time-dependent fields (read by the EQ kernels only)
+  const EqnProg* eq[5];   // programs of V, K, Ax, Ay, Az; null where the field is static
+  const double* vst[3];   // V_TRAP_A: the trap's per-axis terms without 1/2 m |A|^2
+  double t;               // time of the step, Dt * i (src/evolution.cu:408)
+  double org[3];          // equation coordinates x = dx * xi - xMax (src/dynamic.cu:613-615)
+  double dxyz[3];
+  int eqk;                // 1: the step runs the EQ kernels
 };
 
 struct SweepArgs {
@@ -284,16 +305,88 @@ struct WMath<float> {
 };
 
 // ---------------------------------------------------------------------------
+// Time-dependent equation fields, evaluated per point in the EQ kernels.
+// GPUE's operators (src/dynamic.cu:568-600): 1 +, 2 -, 3 *, 4 /, 5 pow with an
+// integer exponent, 6 cos, 7 sin, 8 tan, 9 sqrt, 10 exp, 11 jn, 12 yn.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double eqn_apply(int k, double a, double b) {
+  switch (k) {
+    case 1: return a + b;
+    case 2: return a - b;
+    case 3: return a * b;
+    case 4: return a / b;
+    case 5: return pow(a, (int)b);
+    case 6: return cos(a);
+    case 7: return sin(a);
+    case 8: return tan(a);
+    case 9: return sqrt(a);
+    case 10: return exp(a);
+    case 11: { const int n = (int)a; return n == 0 ? j0(b) : n == 1 ? j1(b) : jn(n, b); }
+    case 12: { const int n = (int)a; return n == 0 ? y0(b) : n == 1 ? y1(b) : yn(n, b); }
+    default: return 0.0;
+  }
+}
+
+// A postfix program with its stack in registers: EQ_REG_DEPTH named slots,
+// shifted on every push and pop so that each index is a compile-time constant
+// (a dynamically indexed array would live in scratch memory).  The host rejects
+// programs that reach a deeper stack.  The op sequence is the same for every
+// lane, so the dispatch branches are uniform.
+// One out-of-line copy per translation unit: the EQ kernels call it per point
+// (inlined E times over every kernel it would multiply their code and compile
+// time for no gain; these kernels are not on the path of static fields).
+constexpr int EQ_REG_DEPTH = 8;
+__device__ __noinline__ double eqn_run(const EqnProg* __restrict__ p, double x, double y, double z, double t) {
+  double s[EQ_REG_DEPTH];
+#pragma unroll
+  for (int k = 0; k < EQ_REG_DEPTH; ++k) s[k] = 0.0;
+  const int n = p->n;
+  for (int i = 0; i < n; ++i) {
+    const int op = p->op[i];
+    if (op < EQ_OP) {
+      const double v = op == EQ_CONST ? p->val[i] : op == EQ_X ? x : op == EQ_Y ? y : op == EQ_Z ? z : t;
+#pragma unroll
+      for (int k = EQ_REG_DEPTH - 1; k > 0; --k) s[k] = s[k - 1];
+      s[0] = v;
+    } else {
+      const double r = eqn_apply(op - EQ_OP, s[1], s[0]);
+#pragma unroll
+      for (int k = 1; k + 1 < EQ_REG_DEPTH; ++k) s[k] = s[k + 1];
+      s[0] = r;
+    }
+  }
+  return s[0];
+}
+
+struct Pt {
+  int x, y, z;
+};
+
+__device__ __forceinline__ double eq_field(const Phys& p, const EqnProg* prog, Pt q) {
+  return eqn_run(prog, p.dxyz[0] * q.x + p.org[0], p.dxyz[1] * q.y + p.org[1], p.dxyz[2] * q.z + p.org[2], p.t);
+}
+
+// A_axis of an A_EQN context: the program where the component depends on t,
+// else its table (components are tables once any of them is time dependent)
+__device__ __forceinline__ double A_eq(const Phys& p, int axis, Pt q, long gid) {
+  if (p.eq[2 + axis]) return eq_field(p, p.eq[2 + axis], q);
+  return p.At[axis] ? p.At[axis][gid] : 0.0;
+}
+
+// V of a V_EQN or V_TRAP_A context
+__device__ __forceinline__ double V_eq(const Phys& p, Pt q, long gid) {
+  if (p.v_kind == V_EQN) return eq_field(p, p.eq[0], q);
+  const double A0 = A_eq(p, 0, q, gid), A1 = A_eq(p, 1, q, gid), A2 = A_eq(p, 2, q, gid);
+  return (p.vst[0][q.x] + p.vst[1][q.y] + p.vst[2][q.z]) + 0.5 * p.mass * (A0 * A0 + A1 * A1 + A2 * A2);
+}
+
+// ---------------------------------------------------------------------------
 // Batched forms of the pointwise operators for the E points a lane holds.  All
 // table loads of the E points are issued before any arithmetic and every
 // branch on a field kind or gstate is taken once, outside the point loop:
 // per-point branches around loads serialise one memory round trip per point.
 // pos(r) gives the (x, y, z) grid indices of register slot r.
 // ---------------------------------------------------------------------------
-struct Pt {
-  int x, y, z;
-};
-
 // The operator arithmetic runs in the working precision T: FP64 sweeps in
 // double; FP32 sweeps in float, with the table values (held in double) rounded
 // once at load.  Only the reductions stay in FP64 in the FP32 form.
@@ -370,9 +463,15 @@ __device__ __forceinline__ void apply_phase_all(cx<T>* v, const T (&arg)[E], int
 // with GV/EV (src/operators.cu:1081,1100) merged into one exponential.
 // In two parts: halfV_vargs gathers V*cV of the lane's points, halfV_apply
 // adds the density term and applies the phase (halfV_all does both in a row).
-template <typename T, int E, typename F>
+template <typename T, int E, bool EQ = false, typename F>
 __device__ __forceinline__ void halfV_vargs(const Phys& p, F pos, T (&arg)[E]) {
-  if (p.v_kind == V_SEP) {
+  if (EQ && (p.v_kind == V_EQN || p.v_kind == V_TRAP_A)) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const Pt q = pos(r);
+      arg[r] = (T)V_eq(p, q, gid_of(p, q.x, q.y, q.z));
+    }
+  } else if (p.v_kind == V_SEP) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const Pt q = pos(r);
@@ -406,19 +505,25 @@ __device__ __forceinline__ void halfV_apply(const Phys& p, cx<T>* v, T (&arg)[E]
   apply_phase_all<T, E, X>(v, arg, p.gstate, 1.0);
 }
 
-template <typename T, int E, bool X = false, typename F>
+template <typename T, int E, bool X = false, bool EQ = false, typename F>
 __device__ __forceinline__ void halfV_all(const Phys& p, cx<T>* v, F pos) {
   T arg[E];
-  halfV_vargs<T, E>(p, pos, arg);
+  halfV_vargs<T, E, EQ>(p, pos, arg);
   halfV_apply<T, E, X>(p, v, arg);
 }
 
 // gauge operator along AXIS: GpA*/EpA* (src/operators.cu:1089-1110) with both
 // 1/sqrt(n) of the axis transforms folded into invn.  gauge_args gathers the
 // exponents A*k*Dt of the lane's points; apply_phase_all applies the phase.
-template <typename T, int E, int AXIS, typename F>
+template <typename T, int E, int AXIS, bool EQ = false, typename F>
 __device__ __forceinline__ void gauge_args(const Phys& p, F pos, T (&arg)[E]) {
-  if (p.a_kind == A_TABLE) {
+  if (EQ && p.a_kind == A_EQN) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const Pt q = pos(r);
+      arg[r] = (T)A_eq(p, AXIS, q, gid_of(p, q.x, q.y, q.z));
+    }
+  } else if (p.a_kind == A_TABLE) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const Pt q = pos(r);
@@ -445,10 +550,10 @@ __device__ __forceinline__ void gauge_args(const Phys& p, F pos, T (&arg)[E]) {
   }
 }
 
-template <typename T, int E, int AXIS, bool X = false, typename F>
+template <typename T, int E, int AXIS, bool X = false, bool EQ = false, typename F>
 __device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double invn) {
   T arg[E];
-  gauge_args<T, E, AXIS>(p, pos, arg);
+  gauge_args<T, E, AXIS, EQ>(p, pos, arg);
   apply_phase_all<T, E, X>(v, arg, p.gstate, invn);
 }
 
@@ -465,9 +570,19 @@ struct KTab<float> {
 // K step: GK/EK (src/operators.cu:1082,1102) with both 1/sqrt(N) of
 // src/evolution.cu:761,777 folded into invN.  For the separable K the operator
 // is the product of per-axis factors (no transcendental per point).
-template <typename T, int E, bool X = false, typename F>
+template <typename T, int E, bool X = false, bool EQ = false, typename F>
 __device__ __forceinline__ void kstep_all(const Phys& p, cx<T>* v, F pos, double invN_) {
   const T invN = (T)invN_;
+  if constexpr (EQ) {
+    if (p.k_kind == K_EQN) {   // as a K table, with the value of the program at time t
+      T arg[E];
+      const T cK = (T)p.cK;
+#pragma unroll
+      for (int r = 0; r < E; ++r) arg[r] = (T)eq_field(p, p.eq[1], pos(r)) * cK;
+      apply_phase_all<T, E, X>(v, arg, p.gstate, invN_);
+      return;
+    }
+  }
   if constexpr (X && sizeof(T) == 4) {
     if (p.k_kind == K_SEP && p.gstate != 0) {
       // factor from the FP64 tables, product in FP64 (see apply_phase_all)
@@ -563,11 +678,11 @@ struct KCol {
   }
 };
 
-template <typename T, int E, int AXIS, bool X = false, typename F>
+template <typename T, int E, int AXIS, bool X = false, bool EQ = false, typename F>
 __device__ __forceinline__ void kstep_col(const Phys& p, cx<T>* v, int xi, int o, F epos, double invN_) {
   if (p.k_kind != K_SEP || (X && sizeof(T) == 4 && p.gstate != 0)) {
-    kstep_all<T, E, X>(p, v, [&](int r) { const int e = epos(r); return AXIS == 1 ? Pt{xi, e, o} : Pt{xi, o, e}; },
-                       invN_);
+    kstep_all<T, E, X, EQ>(p, v, [&](int r) { const int e = epos(r); return AXIS == 1 ? Pt{xi, e, o} : Pt{xi, o, e}; },
+                           invN_);
     return;
   }
   KCol<T, E, AXIS> kc;

@@ -169,7 +169,10 @@ gpue_status gpue_prepare_evolve(gpue_ctx *c, int first_step, int nsteps, int gst
 gpue_status gpue_set_timing(gpue_ctx *c, int on);
 /* Per-sweep-kind accumulated kernel time: kind index, name, launches, ms. */
 gpue_status gpue_timing_get(gpue_ctx *c, int kind, const char **name, long *launches, double *ms);
-/* Algorithmic HBM bytes of one step in the current configuration (psi sweeps). */
+/* HBM bytes of psi one steady-state step moves in the current configuration:
+ * 2 x (bytes of psi) per sweep, averaged over the step parities and, when graphs
+ * are on, over a captured block (real time fuses the sweeps at step boundaries);
+ * *sweeps_per_step receives the sweep count. */
 gpue_status gpue_step_bytes(gpue_ctx *c, int gstate, double *bytes_per_step, double *sweeps_per_step);
 /* Per-kernel cost of the step (no reference counterpart): each distinct sweep
  * of the gstate schedule is launched `reps` times back to back from a graph,

@@ -154,6 +154,25 @@ def ls_fit(ijw, wfc2d):
     return out[: 2 * n].reshape(n, 2)
 
 
+def vort_arrange(cur, prev):
+    """Tracker::vortArrange (src/tracker.cu:234-256) on rows (x, xd, y, yd, winding):
+    for each i, the vortex of cur[i:] nearest to prev[i] (squared distance of the
+    sub-grid positions, converted to int by truncation as GPUE's int dist_t does)
+    is swapped into place i; the first of equal distances wins.  GPUE reads
+    prev[i] for every i < len(cur); here only i < len(prev) are arranged (the
+    rest keep their order), as the drop-in does."""
+    cur = [tuple(r) for r in cur]
+    for i in range(min(len(cur), len(prev))):
+        dist, index = 0x7FFFFFFF, i
+        for j in range(i, len(cur)):
+            dt = int((prev[i][1] - cur[j][1]) * (prev[i][1] - cur[j][1]) +
+                     (prev[i][3] - cur[j][3]) * (prev[i][3] - cur[j][3]))
+            if dist > dt:
+                dist, index = dt, j
+        cur[index], cur[i] = cur[i], cur[index]
+    return cur
+
+
 class Oracle:
     """One GPUE problem on the CPU: grids, fields, operator arrays and psi."""
 
