@@ -252,22 +252,26 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
                   ("1,1,1", {n: (1, 1, 1) for n in tuned})]
     trial = {}
     for name, assign in candidates:
-        for k, c in enumerate(ctxs):
-            c.set_variants(list(assign[int(sizes[k])]))
-        Bt = G.Batch(ctxs)
-        Bt.evolve(10, 0)
-        ctxs[0].sync()
-        t0 = time.perf_counter()
-        Bt.evolve(40, 0)
-        ctxs[0].sync()
-        trial[name] = (time.perf_counter() - t0) / 40
-        Bt.close()
+        for mode in (0, 1):   # one graph per shape group, or one for the whole batch
+            for k, c in enumerate(ctxs):
+                c.set_variants(list(assign[int(sizes[k])]))
+            Bt = G.Batch(ctxs)
+            Bt.set_graph_mode(mode)
+            Bt.evolve(10, 0)
+            ctxs[0].sync()
+            t0 = time.perf_counter()
+            Bt.evolve(40, 0)
+            ctxs[0].sync()
+            trial[f"{name}, graph mode {mode}"] = (time.perf_counter() - t0) / 40
+            Bt.close()
     best = min(trial, key=trial.get)
-    tuned = dict(candidates)[best]
+    best_mode = int(best.rsplit(" ", 1)[1])
+    tuned = dict(candidates)[best.rsplit(",", 1)[0]]
     for k, c in enumerate(ctxs):
         c.set_variants(list(tuned[int(sizes[k])]))
         c.upload(saved[k])
     B = G.Batch(ctxs)
+    B.set_graph_mode(best_mode)
     active = [True] * n_inst
     e_prev = [None] * n_inst
     last_change = [None] * n_inst
@@ -312,7 +316,7 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
             "instance_steps_per_s": round(sum(steps) / el, 1),
             "grid_point_steps_per_s": round(point_steps / el, 1),
             "effective_GBps": round(bytes_total / el / 1e9, 1),
-            "variants_by_size": {int(k): list(v) for k, v in tuned.items()},
+            "variants_by_size": {int(k): list(v) for k, v in tuned.items()}, "graph_mode": best_mode,
             "batch_tuning_us_per_batch_step": {k: round(v * 1e6, 1) for k, v in trial.items()},
             "steps_to_converge": {"min": int(min(steps)), "max": int(max(steps))},
             "convergence_by_omega": by_omega,

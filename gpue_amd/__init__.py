@@ -35,7 +35,7 @@ ABI_SYMBOLS = [
     "gpue_set_field_eqn", "gpue_upload_wfc", "gpue_download_wfc", "gpue_evolve", "gpue_parsum",
     "gpue_energy", "gpue_energy_consistent", "gpue_vortex_marker_2d", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
     "gpue_sync", "gpue_set_ramp", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_prepare_evolve", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
-    "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_evolve", "gpue_batch_energy", "gpue_batch_last_error",
+    "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_set_graph_mode", "gpue_batch_evolve", "gpue_batch_energy", "gpue_batch_last_error",
     "gpue_batch_destroy",
     "gpue_server_create", "gpue_server_submit", "gpue_server_poll", "gpue_server_drain", "gpue_server_clock",
     "gpue_server_destroy",
@@ -122,6 +122,7 @@ def lib():
         L.gpue_eqn_eval.restype = ctypes.c_double
         L.gpue_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
         L.gpue_batch_set_active.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.gpue_batch_set_graph_mode.argtypes = [vp, ctypes.c_int]
         L.gpue_batch_evolve.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.gpue_batch_energy.argtypes = [vp, dp, dp]
         L.gpue_batch_last_error.argtypes = [vp]
@@ -396,6 +397,10 @@ class Batch:
             msg = lib().gpue_batch_last_error(self._h)
             raise GpueError(st, f"{what}: {msg.decode() if msg else ''}")
 
+    def set_graph_mode(self, mode):
+        """0: one graph per shape group (default); 1: one graph for the whole batch."""
+        self._check(lib().gpue_batch_set_graph_mode(self._h, int(mode)), "set_graph_mode")
+
     def set_active(self, index, active):
         self._check(lib().gpue_batch_set_active(self._h, index, 1 if active else 0), "batch_set_active")
 
@@ -435,6 +440,7 @@ class Result(ctypes.Structure):
         ("id", ctypes.c_ulonglong), ("status", ctypes.c_int), ("imag_steps", ctypes.c_int),
         ("real_steps", ctypes.c_int), ("energy", ctypes.c_double), ("mu", ctypes.c_double),
         ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+        ("t_dev_start", ctypes.c_double), ("t_dev_done", ctypes.c_double),
     ]
 
 
@@ -476,7 +482,8 @@ class Server:
             self._keep.pop(int(r.id), None)
             out.append(dict(id=int(r.id), status=int(r.status), imag_steps=int(r.imag_steps),
                             real_steps=int(r.real_steps), energy=float(r.energy), mu=float(r.mu),
-                            t_submit=float(r.t_submit), t_start=float(r.t_start), t_done=float(r.t_done)))
+                            t_submit=float(r.t_submit), t_start=float(r.t_start), t_done=float(r.t_done),
+                            t_dev_start=float(r.t_dev_start), t_dev_done=float(r.t_dev_done)))
         return out
 
     def drain(self, timeout_s=600.0):

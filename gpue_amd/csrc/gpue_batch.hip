@@ -72,6 +72,15 @@ struct gpue_batch {
   std::vector<int> cls;            // parity class: the instance's step parity at phase 0
   std::vector<Group> groups;
   int phase = 0;
+  // graph mode 1: one graph per pair of steps for the whole batch, captured on
+  // `origin` with a branch per shape group (forked to the group streams and
+  // joined back), so a pair of steps of every group is one launch
+  int graph_mode = 0;
+  hipStream_t origin = nullptr;
+  hipEvent_t fork_ev = nullptr;
+  std::vector<hipEvent_t> join_ev;   // per group
+  hipGraph_t wgraph[2] = {nullptr, nullptr};
+  hipGraphExec_t wexec[2] = {nullptr, nullptr};
   double* terms = nullptr;         // pinned: 6 energy terms per instance (gpue_batch_energy)
   std::string err;
 };
@@ -206,11 +215,46 @@ static void drop_batch_graphs(gpue_batch* b) {
       grp.gexec[g] = nullptr;
       grp.graph[g] = nullptr;
     }
+  for (int g = 0; g < 2; ++g) {
+    if (b->wexec[g]) (void)hipGraphExecDestroy(b->wexec[g]);
+    if (b->wgraph[g]) (void)hipGraphDestroy(b->wgraph[g]);
+    b->wexec[g] = nullptr;
+    b->wgraph[g] = nullptr;
+  }
 }
 
 static gpue_status sync_groups(gpue_batch* b) {
   for (Group& g : b->groups)
     if (g.stream) BHIP(b, hipStreamSynchronize(g.stream));
+  if (b->origin) BHIP(b, hipStreamSynchronize(b->origin));
+  return GPUE_OK;
+}
+
+// Graph mode 1: capture a pair of steps of every group as one graph on
+// `origin`: fork to each group's stream, launch its two phases there, join back.
+static gpue_status build_whole_graph(gpue_batch* b, int gstate) {
+  BHIP(b, hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+  gpue_status st = GPUE_OK;
+  hipError_t e = hipEventRecord(b->fork_ev, b->origin);
+  for (size_t k = 0; k < b->groups.size() && st == GPUE_OK && e == hipSuccess; ++k) {
+    Group& grp = b->groups[k];
+    e = hipStreamWaitEvent(grp.stream, b->fork_ev, 0);
+    if (e == hipSuccess) st = launch_group_phase(b, grp, gstate, 0);
+    if (st == GPUE_OK && e == hipSuccess) st = launch_group_phase(b, grp, gstate, 1);
+    if (st == GPUE_OK && e == hipSuccess) e = hipEventRecord(b->join_ev[k], grp.stream);
+    if (st == GPUE_OK && e == hipSuccess) e = hipStreamWaitEvent(b->origin, b->join_ev[k], 0);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(b->origin, &g);
+  if (st != GPUE_OK || e != hipSuccess || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return st != GPUE_OK ? st : bfail(b, GPUE_EHIP, "whole-batch graph capture");
+  }
+  if (hipGraphInstantiate(&b->wexec[gstate], g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return bfail(b, GPUE_EHIP, "whole-batch graph instantiate");
+  }
+  b->wgraph[gstate] = g;
   return GPUE_OK;
 }
 
@@ -220,7 +264,12 @@ void gpue_batch_destroy(gpue_batch* b) {
   if (!b) return;
   for (Group& g : b->groups)
     if (g.stream) (void)hipStreamSynchronize(g.stream);
+  if (b->origin) (void)hipStreamSynchronize(b->origin);
   drop_batch_graphs(b);
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+  for (hipEvent_t ev : b->join_ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (b->origin) (void)hipStreamDestroy(b->origin);
   for (Group& g : b->groups)
     for (int gs = 0; gs < 2; ++gs)
       for (int ph = 0; ph < 2; ++ph)
@@ -236,6 +285,17 @@ void gpue_batch_destroy(gpue_batch* b) {
 }
 
 const char* gpue_batch_last_error(const gpue_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+gpue_status gpue_batch_set_graph_mode(gpue_batch* b, int mode) {
+  if (!b || (mode != 0 && mode != 1)) return GPUE_EINVAL;
+  if (mode != b->graph_mode) {
+    gpue_status st = sync_groups(b);
+    if (st != GPUE_OK) return st;
+    drop_batch_graphs(b);
+    b->graph_mode = mode;
+  }
+  return GPUE_OK;
+}
 
 gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
   if (!ctxs || n <= 0 || !out) return GPUE_EINVAL;
@@ -274,6 +334,17 @@ gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
   }
   for (Group& g : b->groups)
     if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess) {
+      gpue_batch_destroy(b);
+      return GPUE_EHIP;
+    }
+  b->join_ev.assign(b->groups.size(), nullptr);
+  if (hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess) {
+    gpue_batch_destroy(b);
+    return GPUE_EHIP;
+  }
+  for (hipEvent_t& ev : b->join_ev)
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
       gpue_batch_destroy(b);
       return GPUE_EHIP;
     }
@@ -364,6 +435,21 @@ gpue_status gpue_batch_evolve(gpue_batch* b, int nsteps, int gstate) {
     if (st != GPUE_OK) return st;
     b->phase = 0;
     ++s;
+  }
+  if (nsteps - s >= 2 && b->graph_mode == 1) {
+    if (!b->wexec[gstate]) {
+      gpue_status st = build_whole_graph(b, gstate);
+      if (st != GPUE_OK) return st;
+    }
+    // the whole graph runs on `origin` after the groups' earlier work, and their
+    // later work waits for it
+    for (size_t k = 0; k < b->groups.size(); ++k) {
+      BHIP(b, hipEventRecord(b->join_ev[k], b->groups[k].stream));
+      BHIP(b, hipStreamWaitEvent(b->origin, b->join_ev[k], 0));
+    }
+    for (; nsteps - s >= 2; s += 2) BHIP(b, hipGraphLaunch(b->wexec[gstate], b->origin));
+    BHIP(b, hipEventRecord(b->fork_ev, b->origin));
+    for (Group& grp : b->groups) BHIP(b, hipStreamWaitEvent(grp.stream, b->fork_ev, 0));
   }
   if (nsteps - s >= 2) {
     for (Group& grp : b->groups) {

@@ -215,6 +215,15 @@ static bool opens_step(const Sweep& s) {
   return s.row ? s.mode == RM_FWD : s.mode == CM_FWD_V;
 }
 
+static int default_variant(const gpue_ctx* c, int axis);
+
+// An opening sweep that can reduce the previous step's norm partials itself
+// (part_norm needs workgroups of whole waves).
+static bool takes_deferred_norm(const gpue_ctx* c, const Sweep& s) {
+  const int var = c->t_dep ? default_variant(c, s.axis) : s.var >= 0 ? s.var : c->var[s.axis];
+  return opens_step(s) && sweep_full_waves(c->prec, s.row, c->dims[s.axis], var);
+}
+
 static std::string sweep_name(const Sweep& s) {
   static const char* rm[] = {"row_fwd", "row_inv", "row_gauge", "row_full", "row_ksum", "row_inv_fwd",
                              "row_gauge_fwd", "row_k"};
@@ -250,7 +259,6 @@ static gpue_status timing_flush(gpue_ctx* c) {
 }
 
 // Workgroups (= partials written) of a sweep.
-static int default_variant(const gpue_ctx* c, int axis);
 static int sweep_nb(const gpue_ctx* c, const Sweep& s) {
   const int n = c->dims[s.axis];
   const int var = c->t_dep ? default_variant(c, s.axis) : s.var >= 0 ? s.var : c->var[s.axis];
@@ -265,7 +273,8 @@ static int sweep_nb(const gpue_ctx* c, const Sweep& s) {
 // for fewer than 4096 partials (every workgroup of `next` sums all of them)
 // and when `next` is the row sweep that scales its input.
 static bool defer_norm(const gpue_ctx* c, Sweep& last, Sweep& next) {
-  if (c->p.dimnum < 2 || !(last.flags & F_NORM) || !(next.flags & F_SCALE_IN) || !opens_step(next)) return false;
+  if (c->p.dimnum < 2 || !(last.flags & F_NORM) || !(next.flags & F_SCALE_IN) || !takes_deferred_norm(c, next))
+    return false;
   const int nb = sweep_nb(c, last);
   if (nb >= 4096) return false;
   last.flags |= F_DEFER;
@@ -1437,7 +1446,7 @@ static gpue_status evolve_impl(gpue_ctx* c, int first_step, int nsteps, int gsta
     c->t_last = Dt * i;
     schedule(c, gstate, i, c->pending_norm, sched);
     if (c->defer_nb > 0) {   // the previous step left its norm partials to this one
-      if (opens_step(sched[0]) && (sched[0].flags & F_SCALE_IN)) {
+      if (takes_deferred_norm(c, sched[0]) && (sched[0].flags & F_SCALE_IN)) {
         sched[0].flags |= F_SCALE_PART;
         sched[0].part_nb = c->defer_nb;
         c->defer_nb = 0;

@@ -71,6 +71,7 @@ struct Slot {
   gpue_ctx* ctx = nullptr;
   double* host = nullptr;          // pinned: energy terms read back after a chunk
   hipEvent_t ev = nullptr;
+  hipEvent_t tev[2] = {};          // timing events: first GPU work, end of the latest chunk
   int cls = 0;                     // size class: the stream the request runs on
   bool busy = false;
   int gstate = 0;
@@ -91,6 +92,7 @@ struct Pooled {
   gpue_ctx* ctx = nullptr;
   double* host = nullptr;
   hipEvent_t ev = nullptr;
+  hipEvent_t tev[2] = {};
   double bytes = 0.0;
   int cls = 0;
   long last_use = 0;
@@ -115,6 +117,7 @@ struct gpue_server {
   double budget = 0.0, used = 0.0;
   int max_slots = 0;
   hipStream_t streams[NSTREAMS] = {};
+  hipEvent_t ev0 = nullptr;        // timing event at creation: the origin of the device clock
   std::vector<Pooled> pool;
   long pool_clock = 0;
   std::chrono::steady_clock::time_point t0;
@@ -146,7 +149,8 @@ static gpue_status issue_chunk(Slot* s, int nsteps) {
   else s->real_done += nsteps;
   st = energy_launch(c, s->host);
   if (st != GPUE_OK) return st;
-  if (hipEventRecord(s->ev, c->stream) != hipSuccess) return GPUE_EHIP;
+  if (hipEventRecord(s->ev, c->stream) != hipSuccess || hipEventRecord(s->tev[1], c->stream) != hipSuccess)
+    return GPUE_EHIP;
   s->busy = true;
   return GPUE_OK;
 }
@@ -161,6 +165,8 @@ static void shrink_pool(gpue_server* sv, double need) {
       if (sv->pool[j].last_use < sv->pool[k].last_use) k = j;
     Pooled& e = sv->pool[k];
     if (e.ev) (void)hipEventDestroy(e.ev);
+    for (hipEvent_t t : e.tev)
+      if (t) (void)hipEventDestroy(t);
     if (e.host) (void)hipHostFree(e.host);
     gpue_destroy(e.ctx);
     sv->used -= e.bytes;
@@ -182,13 +188,17 @@ static gpue_status admit(gpue_server* sv, const Req& q, const Pooled& reuse, Slo
     s->ctx = reuse.ctx;
     s->host = reuse.host;
     s->ev = reuse.ev;
+    s->tev[0] = reuse.tev[0];
+    s->tev[1] = reuse.tev[1];
     st = ctx_reset(s->ctx, &p);
   } else {
     st = ctx_create(&p, sv->streams[s->cls], &s->ctx);
     if (st == GPUE_OK && (hipHostMalloc((void**)&s->host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-                          hipEventCreateWithFlags(&s->ev, hipEventDisableTiming) != hipSuccess))
+                          hipEventCreateWithFlags(&s->ev, hipEventDisableTiming) != hipSuccess ||
+                          hipEventCreate(&s->tev[0]) != hipSuccess || hipEventCreate(&s->tev[1]) != hipSuccess))
       st = GPUE_EHIP;
   }
+  if (st == GPUE_OK && hipEventRecord(s->tev[0], s->ctx->stream) != hipSuccess) st = GPUE_EHIP;
   if (st == GPUE_OK) {
     gpue_ctx* c = s->ctx;
     const long N = c->N;
@@ -201,6 +211,7 @@ static gpue_status admit(gpue_server* sv, const Req& q, const Pooled& reuse, Slo
                          c->dgrid[1], c->dgrid[2], c->dims[0], c->dims[1], c->dims[2], q.r.seed, q.r.random_phase);
     if (hipGetLastError() != hipSuccess) st = GPUE_EHIP;
     if (st == GPUE_OK) st = gpue_parsum(c);
+    if (st == GPUE_OK && hipEventRecord(s->tev[1], c->stream) != hipSuccess) st = GPUE_EHIP;
   }
   s->t_start = sv->clock();
   s->status = st;
@@ -228,6 +239,14 @@ static void retire(gpue_server* sv, Slot* s) {
     if (st != GPUE_OK) r.status = st;
   }
   r.t_done = sv->clock();
+  if (s->status == GPUE_OK && s->tev[0] && s->tev[1] && sv->ev0) {   // both recorded and complete by now
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, sv->ev0, s->tev[0]) == hipSuccess &&
+        hipEventElapsedTime(&b, sv->ev0, s->tev[1]) == hipSuccess) {
+      r.t_dev_start = 1e-3 * a;
+      r.t_dev_done = 1e-3 * b;
+    }
+  }
   {
     std::lock_guard<std::mutex> lk(sv->mu);
     sv->done.push_back(r);
@@ -244,12 +263,16 @@ static void release_retired(gpue_server* sv) {
       e.ctx = s->ctx;
       e.host = s->host;
       e.ev = s->ev;
+      e.tev[0] = s->tev[0];
+      e.tev[1] = s->tev[1];
       e.bytes = s->bytes;
       e.cls = s->cls;
       e.last_use = ++sv->pool_clock;
       sv->pool.push_back(e);
     } else {   // a failed request's context is not reused
       if (s->ev) (void)hipEventDestroy(s->ev);
+      for (hipEvent_t t : s->tev)
+        if (t) (void)hipEventDestroy(t);
       if (s->host) (void)hipHostFree(s->host);
       if (s->ctx) gpue_destroy(s->ctx);
       sv->used -= s->bytes;
@@ -412,6 +435,12 @@ gpue_status gpue_server_create(int device, double budget, int max_slots, gpue_se
       delete sv;
       return GPUE_EHIP;
     }
+  if (hipEventCreate(&sv->ev0) != hipSuccess || hipEventRecord(sv->ev0, sv->streams[0]) != hipSuccess ||
+      hipEventSynchronize(sv->ev0) != hipSuccess) {
+    for (int k = 0; k < NSTREAMS; ++k) (void)hipStreamDestroy(sv->streams[k]);
+    delete sv;
+    return GPUE_EHIP;
+  }
   sv->worker = std::thread(worker_main, sv);
   *out = sv;
   return GPUE_OK;
@@ -462,6 +491,7 @@ void gpue_server_destroy(gpue_server* sv) {
   shrink_pool(sv, 2.0 * sv->budget + 1.0);   // every pooled context
   for (int k = 0; k < NSTREAMS; ++k)
     if (sv->streams[k]) (void)hipStreamDestroy(sv->streams[k]);
+  if (sv->ev0) (void)hipEventDestroy(sv->ev0);
   delete sv;
 }
 

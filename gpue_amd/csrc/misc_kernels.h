@@ -27,6 +27,7 @@ hipError_t launch_vortex(const void* psi, const unsigned char* mask, int xDim, i
 hipError_t launch_sweep(int prec, bool row, int N, int var, int mode, const SweepArgs& a, hipStream_t s,
                         bool* handled);
 int sweep_blocks(int prec, bool row, int N, int var, int lines_or_xdim, int others);
+bool sweep_full_waves(int prec, bool row, int N, int var);
 hipError_t launch_sweep_batched(int prec, bool row, int N, int var, int mode, const SweepArgs* d, int ninst, int lx,
                                 int others, hipStream_t s, bool* handled);
 

@@ -62,6 +62,30 @@ hipError_t launch_sweep_batched(int prec, bool row, int N, int var, int mode, co
 #undef GPUE_CASE
 }
 
+// Whether the sweep's workgroups are whole waves (64 lanes or a multiple): the
+// deferred norm reduction inside a sweep needs full waves.
+bool sweep_full_waves(int prec, bool row, int N, int var) {
+#define GPUE_WG(N_, V_) (prec == 64 ? ColCfg<double, N_, V_>::WG : ColCfg<float, N_, V_>::WG)
+#define GPUE_CASE(N_)                                                                              \
+  case N_:                                                                                         \
+    if (row) return ((var & 1) ? RowCfg<N_, 1>::WG : RowCfg<N_, 0>::WG) % 64 == 0;              \
+    switch (var) {                                                                                 \
+      case 0: return GPUE_WG(N_, 0) % 64 == 0;                                                     \
+      case 1: return GPUE_WG(N_, 1) % 64 == 0;                                                     \
+      case 2: return GPUE_WG(N_, 2) % 64 == 0;                                                     \
+      case 3: return GPUE_WG(N_, 3) % 64 == 0;                                                     \
+      case 4: return GPUE_WG(N_, 4) % 64 == 0;                                                     \
+      default: return GPUE_WG(N_, 5) % 64 == 0;                                                    \
+    }
+  switch (N) {
+    GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
+    GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
+    default: return false;
+  }
+#undef GPUE_CASE
+#undef GPUE_WG
+}
+
 int sweep_blocks(int prec, bool row, int N, int var, int lines_or_xdim, int others) {
 #define GPUE_COLB(N_, V_) \
   (prec == 64 ? col_blocks<double, N_, V_>(lines_or_xdim, others) : col_blocks<float, N_, V_>(lines_or_xdim, others))

@@ -698,11 +698,16 @@ __device__ __forceinline__ void kstep_col(const Phys& p, cx<T>* v, int xi, int o
 // ---------------------------------------------------------------------------
 template <int NA>
 __device__ __forceinline__ void block_sum(double (&v)[NA], double* sh /* >= NA * 16 */) {
+  // a workgroup smaller than a wave (strided sweeps of 16- and 32-point lines
+  // with narrow tiles) has lanes past its end: the butterfly stays within the
+  // workgroup's lanes (the offsets not taken are >= blockDim.x)
+  const int wl = (int)blockDim.x;
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
     double x = v[a];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    for (int o = 32; o > 0; o >>= 1)
+      if (o < wl) x += __shfl_xor(x, o, 64);
     v[a] = x;
   }
   const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
@@ -794,7 +799,8 @@ __device__ __forceinline__ void fin_sum4(const double* __restrict__ p, int nb, i
 }
 
 // The parSum norm sqrt(sum * scale) of nb < 4096 partials, computed by one full
-// wave (every lane gets it) with the additions of k_finalize in its 256-lane
+// wave (every lane gets it; the host defers a norm only to sweeps whose
+// workgroups are whole waves) with the additions of k_finalize in its 256-lane
 // form, bit for bit: lane l stands for finalize lanes l + 64j (j < 4); each j
 // is one finalize wave, reduced by the same xor butterfly as block_sum (the
 // four run interleaved), and the four wave totals are added in order.

@@ -223,6 +223,11 @@ gpue_status gpue_batch_evolve(gpue_batch *b, int nsteps, int gstate);
  * of gpue_batch_create), evaluated on all instances' streams at once and
  * waited for once: the values are those gpue_energy returns per context. */
 gpue_status gpue_batch_energy(gpue_batch *b, double *energy, double *mu);
+/* How pairs of steps are replayed: 0 (default) one graph per shape group on the
+ * group's stream, launched group by group; 1 one graph for the whole batch, a
+ * branch per group forked onto the group streams, so a pair of steps of every
+ * instance is one launch. */
+gpue_status gpue_batch_set_graph_mode(gpue_batch *b, int mode);
 const char *gpue_batch_last_error(const gpue_batch *b);
 void        gpue_batch_destroy(gpue_batch *b);
 
@@ -251,6 +256,8 @@ typedef struct {
     int imag_steps, real_steps;
     double energy, mu;       /* at the last check */
     double t_submit, t_start, t_done;   /* seconds on the server clock */
+    double t_dev_start, t_dev_done;     /* seconds on the device clock since the server started:
+                                           the request's first GPU work and the end of its last chunk */
 } gpue_result;
 
 typedef struct gpue_server gpue_server;

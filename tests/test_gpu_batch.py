@@ -21,8 +21,9 @@ def make(n, g, om, prec=64, seed=1, gamma=1.0):
     return c, kw, psi
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("prec", [64, 32])
-def test_batch_equals_alone(prec):
+def test_batch_equals_alone(prec, mode):
     specs = [(64, 100.0, 0.3), (128, 300.0, 0.7), (64, 150.0, 0.5), (192, 500.0, 0.9), (128, 50.0, 0.0)]
     batch_ctx, alone = [], []
     for k, (n, g, om) in enumerate(specs):
@@ -37,6 +38,7 @@ def test_batch_equals_alone(prec):
     batch_ctx[2].evolve(1, 0)
     alone[2].evolve(1, 0)
     b = G.Batch(batch_ctx)
+    b.set_graph_mode(mode)   # 1: one graph for the whole batch (a branch per shape group)
     b.evolve(10, 0)
     b.evolve(7, 0)
     b.set_active(4, False)

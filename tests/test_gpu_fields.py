@@ -134,12 +134,12 @@ TD_CASES = {
     "K": (dict(dimnum=2, xDim=32, yDim=32, xMax=6, yMax=6), {G.FIELD_K: "0.5*(x*x+y*y)*(1+0.2*sin(5*t))"}),
     # A(t) of a rotating frame whose rate ramps; V = trap + 1/2 m |A(t)|^2 (GPUE builds V after A)
     "A": (dict(dimnum=2, xDim=32, yDim=32, xMax=6, yMax=6),
-          {G.FIELD_AX: "-y*0.3*(1+2*t)", G.FIELD_AY: "x*0.3*(1+2*t)"}),
+          {G.FIELD_AX: "0-y*0.3*(1+2*t)", G.FIELD_AY: "x*0.3*(1+2*t)"}),
     "VKA": (dict(dimnum=2, xDim=48, yDim=48, xMax=6, yMax=6),
             {G.FIELD_V: "0.5*(x*x+y*y)*(1+0.5*sin(3*t))", G.FIELD_K: "0.5*(x*x+y*y)*(1+0.1*t)",
-             G.FIELD_AX: "-y*0.3*cos(t)", G.FIELD_AY: "x*0.3"}),
+             G.FIELD_AX: "0-y*0.3*cos(t)", G.FIELD_AY: "x*0.3"}),
     "3d-VA": (dict(dimnum=3, xDim=16, yDim=16, zDim=16, xMax=5, yMax=5, zMax=5, omegaZ=1.5),
-              {G.FIELD_V: "0.5*(x*x+y*y+z*z)+0.2*sin(2*t)*x", G.FIELD_AX: "-y*0.4*(1+t)",
+              {G.FIELD_V: "0.5*(x*x+y*y+z*z)+0.2*sin(2*t)*x", G.FIELD_AX: "0-y*0.4*(1+t)",
                G.FIELD_AY: "x*0.4*(1+t)", G.FIELD_AZ: "0.1*x*t"}),
 }
 ORACLE_FIELD = {G.FIELD_V: O.FIELD_V, G.FIELD_K: O.FIELD_K, G.FIELD_AX: O.FIELD_AX, G.FIELD_AY: O.FIELD_AY,
