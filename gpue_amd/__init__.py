@@ -33,7 +33,7 @@ A_ROTATION, A_CONSTANT = 0, 1
 ABI_SYMBOLS = [
     "gpue_params_default", "gpue_create", "gpue_destroy", "gpue_last_error", "gpue_set_field",
     "gpue_set_field_eqn", "gpue_upload_wfc", "gpue_download_wfc", "gpue_evolve", "gpue_parsum",
-    "gpue_energy", "gpue_energy_consistent", "gpue_vortex_marker_2d", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
+    "gpue_energy", "gpue_energy_consistent", "gpue_vortex_marker_2d", "gpue_vortex_marker_2d_noalloc", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
     "gpue_sync", "gpue_set_ramp", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_prepare_evolve", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
     "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_set_graph_mode", "gpue_batch_evolve", "gpue_batch_energy", "gpue_batch_last_error",
     "gpue_batch_destroy",
@@ -101,6 +101,7 @@ def lib():
         L.gpue_energy.argtypes = [vp, dp, dp]
         L.gpue_energy_consistent.argtypes = [vp, dp, dp]
         L.gpue_vortex_marker_2d.argtypes = [vp, ctypes.c_double, ip, ip]
+        L.gpue_vortex_marker_2d_noalloc.argtypes = [vp, ctypes.c_double, ip, ip]
         L.gpue_vortex_list_2d.argtypes = [vp, ctypes.c_double, ip, ctypes.c_int, ip]
         L.gpue_get_grid.argtypes = [vp, ctypes.c_int, dp]
         L.gpue_set_g.argtypes = [vp, ctypes.c_double]

@@ -119,6 +119,14 @@ gpue_status gpue_energy_consistent(gpue_ctx *c, double *energy, double *mu);
 /* Tracker::findVortex (src/tracker.cu:102-161) on the current 2D psi:
  * marker_out (xDim*xDim ints, overwritten) and the count of hits. */
 gpue_status gpue_vortex_marker_2d(gpue_ctx *c, double mask_radius, int *marker_out, int *found);
+/* No counterpart in the reference: gpue_vortex_marker_2d for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same markers and count; its scratch
+ * (plaquette mask, candidates, marker) is a fixed device array of the library,
+ * one per GPU, held until the process ends, for square grids up to 1024^2
+ * (larger: GPUE_EINVAL).  Calls on one GPU share that array, so calls that
+ * overlap on one GPU race; each call returns after its results are copied. */
+gpue_status gpue_vortex_marker_2d_noalloc(gpue_ctx *c, double mask_radius, int *marker_out, int *found);
 /* findVortex + vortPos (src/tracker.cu:212-229): raster-ordered (i, j, winding). */
 gpue_status gpue_vortex_list_2d(gpue_ctx *c, double mask_radius, int *ijw, int max_vortices, int *n_vortices);
 
