@@ -347,9 +347,10 @@ gpue_status gpue::launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void
   if (ksum || ((s.flags & F_NORM) && !(s.flags & F_DEFER))) {
     const int nb = sweep_nb(c, s);
     const double dV = c->d[0] * c->d[1] * c->d[2];
-    // KSUM sums land in results[1] (kinetic) or, with F_ASUM, results[6 + axis]
-    e = ksum ? launch_finalize(c->partials, nb, 1, 0, s.invN, c->results + ((s.flags & F_ASUM) ? 6 + s.axis : 1),
-                               c->stream)
+    // KSUM sums land in results[1] (kinetic), with F_ASUM in results[6 + axis],
+    // with F_KAX in results[9 + axis]
+    const int slot = (s.flags & F_ASUM) ? 6 + s.axis : (s.flags & F_KAX) ? 9 + s.axis : 1;
+    e = ksum ? launch_finalize(c->partials, nb, 1, 0, s.invN, c->results + slot, c->stream)
              : launch_finalize(c->partials, nb, 1, 1, dV, c->results + 0, c->stream);
     if (e != hipSuccess) return fail(c, GPUE_EHIP, std::string("finalize: ") + hipGetErrorString(e));
   }
@@ -976,20 +977,31 @@ static gpue_status energy_phys(gpue_ctx* c, Phys* out, bool fill) {
 }
 
 gpue_status gpue::energy_launch(gpue_ctx* c, double* host_dst) {
-  gpue_status st = apply_pending(c);
-  if (st != GPUE_OK) return st;
   const gpue_params& p = c->p;
   Phys ph;
-  st = energy_phys(c, &ph, true);
+  gpue_status st = energy_phys(c, &ph, true);
   if (st != GPUE_OK) return st;
-  const double invN = 1.0 / (double)c->N;
-  // kinetic term by Parseval over the full transform
-  if (p.dimnum == 1) {
-    st = launch(c, {true, 0, RM_KSUM, 0u, invN}, ph, c->psi, c->psi);
+  // A separable K (GPUE's simple_K, src/operators.cu:460-469) is summed one axis
+  // at a time: by Parseval along the other axes, sum_k K_a(k_a) |psi_hat|^2 / N
+  // equals sum over the mixed representation (transformed along a only) of
+  // K_a(k_a) |psi_a|^2 / n_a.  Each axis is then one read-only sweep of psi, and
+  // a pending renormalisation is divided out of the sums instead of applied.
+  const bool sep = ph.k_kind == K_SEP;
+  if (sep) {
+    for (int a = 0; a < p.dimnum && st == GPUE_OK; ++a)
+      st = launch(c, {a == 0, a, a == 0 ? (int)RM_KSUM : (int)CM_KSUM, F_KAX, 1.0 / (double)c->dims[a]}, ph, c->psi,
+                  c->psi);
   } else {
-    st = launch(c, {true, 0, RM_FWD, 0u, 0.0}, ph, c->psi, c->tmp);
-    if (st == GPUE_OK && p.dimnum == 3) st = launch(c, {false, 1, CM_FWD, 0u, 0.0}, ph, c->tmp, c->tmp);
-    if (st == GPUE_OK) st = launch(c, {false, p.dimnum == 2 ? 1 : 2, CM_KSUM, 0u, invN}, ph, c->tmp, c->tmp);
+    // a K table: Parseval over the full transform of the normalised psi
+    st = apply_pending(c);
+    const double invN = 1.0 / (double)c->N;
+    if (st == GPUE_OK && p.dimnum == 1) {
+      st = launch(c, {true, 0, RM_KSUM, 0u, invN}, ph, c->psi, c->psi);
+    } else if (st == GPUE_OK) {
+      st = launch(c, {true, 0, RM_FWD, 0u, 0.0}, ph, c->psi, c->tmp);
+      if (st == GPUE_OK && p.dimnum == 3) st = launch(c, {false, 1, CM_FWD, 0u, 0.0}, ph, c->tmp, c->tmp);
+      if (st == GPUE_OK) st = launch(c, {false, p.dimnum == 2 ? 1 : 2, CM_KSUM, 0u, invN}, ph, c->tmp, c->tmp);
+    }
   }
   if (st != GPUE_OK) return st;
   const int gauge = p.corotating && p.dimnum > 1;
@@ -999,6 +1011,9 @@ gpue_status gpue::energy_launch(gpue_ctx* c, double* host_dst) {
                                : launch_energy_pos<float>(c->psi, ph, gauge, p.hbar, c->d[0], c->d[1], c->d[2],
                                                           c->partials, &nb, c->stream);
   if (e == hipSuccess) e = launch_finalize(c->partials, nb, 4, 0, 1.0, c->results + 2, c->stream);
+  // results[1..5] of the normalised psi: kinetic from the per-axis sums, and
+  // every term divided by the pending norm (results[0], 1.0 when none)
+  if (e == hipSuccess) e = launch_energy_fold(c->results, sep ? p.dimnum : 0, c->stream);
   if (e == hipSuccess && host_dst)
     e = hipMemcpyAsync(host_dst, c->results, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
@@ -1059,7 +1074,7 @@ gpue_status gpue_energy_consistent(gpue_ctx* c, double* energy, double* mu) {
   if (p.corotating && p.dimnum > 1) {
     double s = 0.0;
     for (int a = 0; a < naxes; ++a) s += r[6 + a];
-    r[4] = p.hbar * s;
+    r[4] = p.hbar * s / (r[0] * r[0]);   // psi may still carry a pending norm
   }
   double E = 0, M = 0;
   energy_combine(c, r, &E, &M);

@@ -41,6 +41,7 @@ struct FinArgs {
 };
 hipError_t launch_finalize_batched(const FinArgs* d, int ninst, hipStream_t s);
 hipError_t launch_set_double(double* p, double v, hipStream_t s);
+hipError_t launch_energy_fold(double* results, int naxes, hipStream_t s);
 hipError_t launch_copy16(const void* in, void* out, size_t bytes, hipStream_t s);
 hipError_t launch_math_eval(int fn, const double* x, double* y, long n, hipStream_t s);
 // variant = plan (bit 0: 0 wide lines, 1 narrow lines) + 2 * narrow, where the

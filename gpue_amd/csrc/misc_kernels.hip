@@ -263,6 +263,30 @@ hipError_t launch_set_double(double* p, double v, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Energy terms of psi / norm from sums over the stored psi: results[0] is the
+// pending norm (1.0 when none), results[2..5] the position sums (V|psi|^2,
+// |psi|^4, gauge term, |psi|^2); with naxes > 0 the kinetic sum is the total of
+// the per-axis sums results[9..9+naxes-1], else results[1] already holds it.
+__global__ void k_energy_fold(double* r, int naxes) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double inv2 = 1.0 / (r[0] * r[0]);
+  double kin = r[1];
+  if (naxes > 0) {
+    kin = r[9];
+    for (int a = 1; a < naxes; ++a) kin += r[9 + a];
+  }
+  r[1] = kin * inv2;
+  r[2] *= inv2;
+  r[3] *= inv2 * inv2;
+  r[4] *= inv2;
+  r[5] *= inv2;
+}
+
+hipError_t launch_energy_fold(double* results, int naxes, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_fold, dim3(1), dim3(64), 0, s, results, naxes);
+  return hipGetLastError();
+}
+
 // the FP64 transcendentals of the fused operators, pointwise (verification)
 __global__ void k_math_eval(int fn, const double* __restrict__ x, double* __restrict__ y, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {

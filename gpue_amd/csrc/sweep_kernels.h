@@ -149,7 +149,9 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
         for (int r = 0; r < E; ++r) {
           const int xi = bpos<N, TPL, RFL>(tid, r);
           const long gid = gid_of(p, xi, yi, zi);
-          const double w = (flags & F_ASUM) ? A_val(p, 0, xi, yi, zi, gid) * p.pxg[xi] : K_val(p, xi, yi, zi, gid);
+          const double w = (flags & F_ASUM)  ? A_val(p, 0, xi, yi, zi, gid) * p.pxg[xi]
+                           : (flags & F_KAX) ? p.ks[0][xi]
+                                             : K_val(p, xi, yi, zi, gid);
           acc += w * abs2d(v[r]);
         }
       }
@@ -301,7 +303,8 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
           const long gid = gid_of(p, xi, yi, zi);
           const double w = (flags & F_ASUM)
                                ? A_val(p, a.axis, xi, yi, zi, gid) * (a.axis == 1 ? p.pyg[yi] : p.pzg[zi])
-                               : K_val(p, xi, yi, zi, gid);
+                           : (flags & F_KAX) ? (a.axis == 1 ? p.ks[1][yi] : p.ks[2][zi])
+                                             : K_val(p, xi, yi, zi, gid);
           acc += w * abs2d(v[r]);
         }
       }

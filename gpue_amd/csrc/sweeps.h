@@ -57,6 +57,8 @@ enum : unsigned {
                        // the next sweep (F_SCALE_PART)
   F_ASUM = 128u,       // KSUM modes: weight A_axis * k_axis (the gauge term of the
                        // evolved Hamiltonian) instead of K
+  F_KAX = 256u,        // KSUM modes: weight ks[axis][k_axis], this axis's part of a
+                       // separable K (Parseval along the other axes needs no transform)
 };
 
 // Time-dependent equation program (postfix): CONST pushes val, X/Y/Z/T push a
