@@ -85,21 +85,24 @@ def run_schedule(ctx, imag, real):
 # (each scaled by its reading for a plain 256 MiB device copy: 2.0 and 1.0), per
 # kernel instance and workload; the passes and tables are in profiles/.
 MEASURED_TRAFFIC = {
-    "gpue::col_sweep<double, 512, 5, 2>@512sq": 8804762,
-    "gpue::col_sweep<double, 512, 5, 3>@512sq": 8789504,
-    "gpue::row_sweep<double, 512, 3, 0>@512sq": 8715878,
-    "gpue::row_sweep<double, 512, 3, 1>@512sq": 8894259,
-    "gpue::row_sweep<double, 512, 3, 2>@512sq": 8808243,
-    "gpue::col_sweep<double, 256, 1, 0>@256cube": 536974439,
-    "gpue::col_sweep<double, 256, 1, 1>@256cube": 536981941,
-    "gpue::col_sweep<double, 256, 2, 2>@256cube": 537384725,
-    "gpue::col_sweep<float, 256, 0, 0>@256cube": 268574894,
-    "gpue::col_sweep<float, 256, 0, 1>@256cube": 268569892,
-    "gpue::col_sweep<float, 256, 0, 2>@256cube": 268867855,
-    "gpue::row_sweep<double, 256, 3, 0>@256cube": 537111506,
-    "gpue::row_sweep<double, 256, 3, 1>@256cube": 537705260,
-    "gpue::row_sweep<float, 256, 0, 0>@256cube": 268643041,
-    "gpue::row_sweep<float, 256, 0, 1>@256cube": 268839678,
+    "gpue::col_sweep<double, 512, 5, 2>@512sq": 8813880,
+    "gpue::col_sweep<double, 512, 5, 3>@512sq": 8791408,
+    "gpue::col_sweep<double, 512, 5, 5>@512sq": 8711989,
+    "gpue::col_sweep<double, 512, 5, 6>@512sq": 8977974,
+    "gpue::row_sweep<double, 512, 3, 0>@512sq": 8699992,
+    "gpue::row_sweep<double, 512, 3, 1>@512sq": 8989775,
+    "gpue::row_sweep<double, 512, 3, 2>@512sq": 8799669,
+    "gpue::row_sweep<double, 512, 3, 7>@512sq": 8833750,
+    "gpue::col_sweep<double, 256, 0, 0>@256cube": 537014011,
+    "gpue::col_sweep<double, 256, 0, 1>@256cube": 537005628,
+    "gpue::col_sweep<double, 256, 2, 2>@256cube": 537390597,
+    "gpue::col_sweep<float, 256, 0, 0>@256cube": 268557347,
+    "gpue::col_sweep<float, 256, 0, 1>@256cube": 268550919,
+    "gpue::col_sweep<float, 256, 0, 2>@256cube": 268820343,
+    "gpue::row_sweep<double, 256, 1, 0>@256cube": 537112246,
+    "gpue::row_sweep<double, 256, 1, 1>@256cube": 538254835,
+    "gpue::row_sweep<float, 256, 0, 0>@256cube": 268637783,
+    "gpue::row_sweep<float, 256, 0, 1>@256cube": 268944100,
 }
 
 RM = {"row_fwd": 0, "row_inv": 1, "row_gauge": 2, "row_full": 3, "row_ksum": 4, "row_inv_fwd": 5,
