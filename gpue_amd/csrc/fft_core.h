@@ -222,6 +222,10 @@ GPUE_PLAN(512, 0, 16, 3, 16, 16, 2)
 GPUE_PLAN(512, 1, 8, 3, 8, 8, 8)
 GPUE_PLAN(1024, 0, 16, 3, 16, 16, 4)
 GPUE_PLAN(1024, 1, 8, 4, 8, 8, 8, 2)
+GPUE_PLAN(2048, 0, 16, 3, 16, 16, 8)
+GPUE_PLAN(2048, 1, 8, 4, 8, 8, 8, 4)
+GPUE_PLAN(4096, 0, 16, 3, 16, 16, 16)
+GPUE_PLAN(4096, 1, 8, 4, 8, 8, 8, 8)
 #undef GPUE_PLAN
 
 enum { ORD_FWD = 0, ORD_REV = 1 };

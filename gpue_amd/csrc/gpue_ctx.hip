@@ -38,7 +38,7 @@ static bool plan_radices(int n, int plan, std::vector<int>& R) {
 #define GPUE_RAD(N_) \
   case N_: plan ? radices_of<N_, 1>(R) : radices_of<N_, 0>(R); return true;
     GPUE_RAD(16) GPUE_RAD(32) GPUE_RAD(48) GPUE_RAD(64) GPUE_RAD(96) GPUE_RAD(128) GPUE_RAD(192) GPUE_RAD(256)
-    GPUE_RAD(384) GPUE_RAD(512) GPUE_RAD(1024)
+    GPUE_RAD(384) GPUE_RAD(512) GPUE_RAD(1024) GPUE_RAD(2048) GPUE_RAD(4096)
 #undef GPUE_RAD
     default: return false;
   }
@@ -560,7 +560,7 @@ static gpue_status check_params(const gpue_params& p, std::string& msg) {
   for (int a = 0; a < p.dimnum; ++a)
     if (!valid_dim(dims[a]))
       return msg = "unsupported grid size " + std::to_string(dims[a]) +
-                   " (supported: 16 32 48 64 96 128 192 256 384 512 1024)",
+                   " (supported: 16 32 48 64 96 128 192 256 384 512 1024 2048 4096)",
              GPUE_EUNSUPPORTED;
   if (!(p.hbar > 0) || !(p.mass > 0) || !(p.xMax > 0) || (p.dimnum > 1 && !(p.yMax > 0)) ||
       (p.dimnum > 2 && !(p.zMax > 0)))

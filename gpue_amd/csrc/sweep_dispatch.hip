@@ -12,12 +12,14 @@ namespace gpue {
   GPUE_DECL1(sweep_f64_##N_##_v0) GPUE_DECL1(sweep_f32_##N_##_v0)                         \
   GPUE_DECL1(sweep_f64_##N_##_v1) GPUE_DECL1(sweep_f32_##N_##_v1)
 GPUE_DECL(16) GPUE_DECL(32) GPUE_DECL(48) GPUE_DECL(64) GPUE_DECL(96) GPUE_DECL(128) GPUE_DECL(192) GPUE_DECL(256)
-GPUE_DECL(384) GPUE_DECL(512) GPUE_DECL(1024)
+GPUE_DECL(384) GPUE_DECL(512) GPUE_DECL(1024) GPUE_DECL(2048)
+GPUE_DECL(4096)
 #undef GPUE_DECL
 
 bool sweep_supported(int N) {
   switch (N) {
     case 16: case 32: case 48: case 64: case 96: case 128: case 192: case 256: case 384: case 512: case 1024:
+    case 2048: case 4096:
       return true;
     default: return false;
   }
@@ -37,7 +39,8 @@ hipError_t launch_sweep(int prec, bool row, int N, int var, int mode, const Swee
     return prec == 64 ? sweep_f64_##N_##_v1(row, nw, mode, a, s) : sweep_f32_##N_##_v1(row, nw, mode, a, s);
   switch (N) {
     GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
-    GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
+    GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024) GPUE_CASE(2048)
+    GPUE_CASE(4096)
     default: *handled = false; return hipErrorInvalidValue;
   }
 #undef GPUE_CASE
@@ -56,7 +59,8 @@ hipError_t launch_sweep_batched(int prec, bool row, int N, int var, int mode, co
                       : sweep_f32_##N_##_v1_b(row, nw, mode, d, ninst, lx, others, s);
   switch (N) {
     GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
-    GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
+    GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024) GPUE_CASE(2048)
+    GPUE_CASE(4096)
     default: *handled = false; return hipErrorInvalidValue;
   }
 #undef GPUE_CASE
@@ -79,7 +83,8 @@ bool sweep_full_waves(int prec, bool row, int N, int var) {
     }
   switch (N) {
     GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
-    GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
+    GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024) GPUE_CASE(2048)
+    GPUE_CASE(4096)
     default: return false;
   }
 #undef GPUE_CASE
@@ -102,7 +107,8 @@ int sweep_blocks(int prec, bool row, int N, int var, int lines_or_xdim, int othe
     }
   switch (N) {
     GPUE_CASE(16) GPUE_CASE(32) GPUE_CASE(48) GPUE_CASE(64) GPUE_CASE(96) GPUE_CASE(128) GPUE_CASE(192) GPUE_CASE(256)
-    GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024)
+    GPUE_CASE(384) GPUE_CASE(512) GPUE_CASE(1024) GPUE_CASE(2048)
+    GPUE_CASE(4096)
     default: return 0;
   }
 #undef GPUE_CASE

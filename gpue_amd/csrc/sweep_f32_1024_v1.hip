@@ -1,3 +1,3 @@
-// generated: sweep kernels for N=1024, variant 1, float
+// written by gen_units.py: sweep kernels for N=1024, plan 1, float
 #include "sweep_launch.h"
 GPUE_SWEEP_TU(float, 1024, 1, sweep_f32_1024_v1)

@@ -1,3 +1,3 @@
-// generated: sweep kernels for N=64, variant 1, float
+// written by gen_units.py: sweep kernels for N=64, plan 1, float
 #include "sweep_launch.h"
 GPUE_SWEEP_TU(float, 64, 1, sweep_f32_64_v1)

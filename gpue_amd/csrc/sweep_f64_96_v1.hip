@@ -1,3 +1,3 @@
-// generated: sweep kernels for N=96, variant 1, double
+// written by gen_units.py: sweep kernels for N=96, plan 1, double
 #include "sweep_launch.h"
 GPUE_SWEEP_TU(double, 96, 1, sweep_f64_96_v1)

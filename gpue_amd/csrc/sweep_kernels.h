@@ -37,7 +37,10 @@ struct ColCfg {
   static constexpr int TCmin = 128 / (2 * (int)sizeof(T));
   static constexpr int TCa = 256 / TPL > TCmin ? 256 / TPL : TCmin;
   static constexpr int TCb = TCa > 64 ? 64 : TCa;
-  static constexpr int TC0 = TPL * TCb > 1024 ? 1024 / TPL : TCb;
+  static constexpr int TCw = TPL * TCb > 1024 ? 1024 / TPL : TCb;
+  // the tile's LDS (TC lines of N points) within 128 KiB of the 160 KiB per workgroup
+  static constexpr int TCm = 131072 / (N * 2 * (int)sizeof(T));
+  static constexpr int TC0 = TCw < TCm ? TCw : TCm;
   static constexpr int TC = (TC0 >> NARROW) > 0 ? (TC0 >> NARROW) : 1;
   static constexpr int G = TC0 / TC;   // tiles per default-width tile
   static constexpr int WG = TPL * TC;

@@ -43,7 +43,7 @@ def test_params_default_matches_gpue_parser():
     (dict(dimnum=4), G.GPUE_EINVAL),
     (dict(precision=16), G.GPUE_EINVAL),
     (dict(xDim=100, yDim=100), G.GPUE_EUNSUPPORTED),
-    (dict(xDim=2048, yDim=2048), G.GPUE_EUNSUPPORTED),
+    (dict(xDim=8192, yDim=8192), G.GPUE_EUNSUPPORTED),
     (dict(xMax=-1.0), G.GPUE_EINVAL),
     (dict(hbar=0.0), G.GPUE_EINVAL),
 ])

@@ -64,6 +64,16 @@ CASES = [
                          corotating=1, omega=0.3)),
     ("3d-384z", 3, dict(dimnum=3, xDim=32, yDim=48, zDim=384, xMax=6, yMax=6, zMax=20, g=60, gpe=1, omegaZ=0.3,
                         corotating=1, omega=0.4)),
+    # the largest lines: 2048 (radix 16,16,8 / 8,8,8,4) and 4096 (16,16,16 / 8,8,8,8), along x (rows),
+    # y (strided, stride xDim) and z (strided, stride xDim*yDim)
+    ("2d-2048", 2, dict(dimnum=2, xDim=2048, yDim=2048, xMax=14, yMax=14, g=600, gpe=1, corotating=1, omega=0.7)),
+    ("2d-4096x64", 2, dict(dimnum=2, xDim=4096, yDim=64, xMax=40, yMax=6, g=100, gpe=1, corotating=1, omega=0.3)),
+    ("2d-64x4096", 2, dict(dimnum=2, xDim=64, yDim=4096, xMax=6, yMax=40, g=100, gpe=1, corotating=1, omega=0.3,
+                           gammaY=0.05)),
+    ("3d-2048z", 3, dict(dimnum=3, xDim=16, yDim=16, zDim=2048, xMax=4, yMax=4, zMax=60, g=40, gpe=1, omegaZ=0.02,
+                         corotating=1, omega=0.3)),
+    ("3d-4096z", 3, dict(dimnum=3, xDim=16, yDim=16, zDim=4096, xMax=4, yMax=4, zMax=80, g=40, gpe=1, omegaZ=0.01,
+                         corotating=1, omega=0.3)),
 ]
 
 
@@ -256,10 +266,13 @@ def test_graph_replay_matches_eager(gstate, case):
     b.close()
 
 
+VARIANT_CASES = ("1d-gauge", "2d-gauge", "3d-gauge", "2d-48", "3d-96", "3d-longz", "2d-4096x64", "2d-64x4096",
+                 "3d-2048z")
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
-@pytest.mark.parametrize("name,dim,kw", [c for c in CASES if c[0] in ("1d-gauge", "2d-gauge", "3d-gauge", "2d-48",
-                                                                      "3d-96", "3d-longz")],
-                         ids=["1d-gauge", "2d-gauge", "3d-gauge", "2d-48", "3d-96", "3d-longz"])
+@pytest.mark.parametrize("name,dim,kw", [c for c in CASES if c[0] in VARIANT_CASES],
+                         ids=[c[0] for c in CASES if c[0] in VARIANT_CASES])
 def test_every_kernel_variant(name, dim, kw, variant):
     # the autotuner may pick any plan and strided tile width per axis: each
     # must meet the FP64 bar, imaginary and real time

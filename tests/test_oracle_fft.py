@@ -36,6 +36,17 @@ def test_fft_golden_vectors(golden):
     assert np.max(np.abs(a - g["fwd_nd"])) <= 1e-12 * np.max(np.abs(g["fwd_nd"]))
 
 
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_fft_long_lines_vs_numpy(n):
+    # the longest lines the HIP path takes: pinned to numpy.fft (pocketfft)
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    for sign, ref in ((-1, np.fft.fft(a, axis=1)), (1, np.fft.ifft(a, axis=1) * n)):
+        b = a.copy()
+        O.fft_lines(b, n, 3, 1, n, sign)
+        assert np.max(np.abs(b - ref)) <= 1e-12 * np.max(np.abs(ref)), (n, sign)
+
+
 def test_parseval_and_roundtrip():
     rng = np.random.default_rng(3)
     a = rng.standard_normal((16, 24, 32)) + 1j * rng.standard_normal((16, 24, 32))
