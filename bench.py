@@ -213,6 +213,42 @@ def cpu_baseline(blocks=5, block_steps=50):
             "physical_cores_lscpu": physical_cores(), "cpus_allowed": share}
 
 
+def cpu_other_configs():
+    """The FP64 oracle on the 256^2 ground state (2D, g=500) and the 256^3 ground
+    state (3D, g=5000), imaginary time, with every thread this process may use
+    and with one: steps/s as the median of timed blocks.  256^2: 5 blocks of 50
+    steps each; 256^3 (about a second per step on 16 threads): 3 blocks of 2 steps
+    on all threads and 1 step on one core, so the default run stays short."""
+    from oracle import oracle as O
+
+    share = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(int(env), share) if env.isdigit() and int(env) > 0 else share
+    out = {}
+    for name, cfg, plan in (("256sq_ground_state", CFG1, {threads: (5, 50), 1: (5, 50)}),
+                            ("256cube_ground_state", CFG3, {threads: (3, 2), 1: (1, 1)})):
+        orc = O.Oracle(**cfg)
+        shape = tuple(int(cfg[k]) for k in ("xDim", "yDim", "zDim")[: cfg["dimnum"]])
+        grids = [orc.grid(a) for a in range(cfg["dimnum"])]
+        psi = seeded_gaussian(shape, grids, 1, False, 0.01)
+        psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * orc.dV)
+        res = {}
+        for nt, (blocks, steps) in plan.items():
+            O.set_threads(nt)
+            orc.set_wfc(psi)
+            per = []
+            for b in range(blocks):
+                t0 = time.perf_counter()
+                for i in range(steps):
+                    orc.step(b * steps + i, 0)
+                per.append(steps / (time.perf_counter() - t0))
+            res["threads_%d" % nt] = {"steps_per_s": round(float(np.median(per)), 3),
+                                      "sample": f"median of {blocks} blocks of {steps} imaginary-time steps"}
+        out[name] = res
+    O.set_threads(threads)
+    return out
+
+
 def config4_specs(n_inst=64, seed=4):
     """The batched sweep's instances: (sizes, [(problem, psi0 seed)]).  Grids
     are a seeded mix of 128^2-512^2, g ~ U[100,2000], Omega ~ U[0,0.95] through
@@ -371,11 +407,27 @@ def serving_stream(G, n_req=48, rate=200.0, seed=5, budget=250e9, slots=8):
     kinds = {}
     for (k, p, kw, kind, n, prec) in reqs:
         kinds[kind] = kinds.get(kind, 0) + 1
+
+    def pct(a):
+        a = np.asarray(a, dtype=float)
+        return {"p50": round(float(np.percentile(a, 50)), 2), "p99": round(float(np.percentile(a, 99)), 2),
+                "max": round(float(a.max()), 2)}
+
+    # per request class: latency (submit to result) and device time (first kernel to the
+    # end of the last chunk, from the request's own events)
+    info = {k: (kind, n, prec) for (k, p, kw, kind, n, prec) in reqs}
+    by_class = {}
+    for r in res:
+        kind, n, prec = info[r["id"]]
+        key = f"{kind}_{n}_fp{prec}"
+        e = by_class.setdefault(key, {"lat": [], "dev": []})
+        e["lat"].append((r["t_done"] - r["t_submit"]) * 1e3)
+        e["dev"].append((r["t_dev_done"] - r["t_dev_start"]) * 1e3)
+    classes = {k: {"requests": len(v["lat"]), "latency_ms": pct(v["lat"]), "device_ms": pct(v["dev"])}
+               for k, v in sorted(by_class.items())}
     return {"requests": n_req, "mean_rate_req_s": rate, "failed": len(bad), "mix": kinds,
             "wall_s": round(el, 3), "throughput_req_s": round(len(res) / el, 2),
-            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
-                           "p99": round(float(np.percentile(lat, 99)), 2),
-                           "max": round(float(lat.max()), 2)},
+            "latency_ms": pct(lat), "by_class": classes,
             "budget_GB": budget / 1e9, "max_slots": slots}
 
 
@@ -561,6 +613,8 @@ def main():
         log(f"CPU baseline: {cpu['value']} steps/s on {cpu['cores']} threads, {cpu['one_core']['value']} on one")
         if sps == sps:
             extra["speedup_vs_cpu_baseline"] = round(sps / cpu["value"], 1)
+        extra["cpu_oracle_other_configs"] = cpu_other_configs()
+        log("CPU oracle, other configs:", json.dumps(extra["cpu_oracle_other_configs"]))
 
     out = {
         "metric": "split-op steps/s (512^2 rotating condensate, FP64, gauge Omega=0.9)",
