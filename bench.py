@@ -266,7 +266,7 @@ def config4_specs(n_inst=64, seed=4):
     return sizes, specs
 
 
-def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
+def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8, force_mode=None, force_variants=None):
     """64 independent 2D imaginary-time solves on uneven grids, FP32 bulk with
     FP64 norm/energy, advanced together through one batch (one launch per sweep
     per shape, two steps per graph replay) until each instance's energy changes
@@ -274,43 +274,28 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
     the noisy Gaussian exp(-r^2/2)(1 + 0.01 U[-1,1]), seeded per instance."""
     sizes, specs = config4_specs(n_inst, seed)
     ctxs = []
-    tuned = {}
     for k, (cfg, psi_seed) in enumerate(specs):
-        n = int(sizes[k])
         ctx, _ = make_ctx(G, 32, cfg, seed=psi_seed, phase=False, noise=0.01)
-        if n not in tuned:
-            tuned[n] = ctx.autotune()
-        else:
-            ctx.set_variants(tuned[n])
+        if force_variants is not None:
+            ctx.set_variants(list(force_variants))
         ctxs.append(ctx)
-    # batch tuning: the per-size choice serves a lone instance (latency bound);
-    # in the batch the groups share the device, so a few whole-batch assignments
-    # are timed over 40 steps and the fastest kept (states restored after)
-    saved = [c.download() for c in ctxs]
-    candidates = [("per-size autotune", dict(tuned)), ("3,1,1", {n: (3, 1, 1) for n in tuned}),
-                  ("1,1,1", {n: (1, 1, 1) for n in tuned})]
-    trial = {}
-    for name, assign in candidates:
-        for mode in (0, 1):   # one graph per shape group, or one for the whole batch
-            for k, c in enumerate(ctxs):
-                c.set_variants(list(assign[int(sizes[k])]))
-            Bt = G.Batch(ctxs)
-            Bt.set_graph_mode(mode)
-            Bt.evolve(10, 0)
-            ctxs[0].sync()
-            t0 = time.perf_counter()
-            Bt.evolve(40, 0)
-            ctxs[0].sync()
-            trial[f"{name}, graph mode {mode}"] = (time.perf_counter() - t0) / 40
-            Bt.close()
-    best = min(trial, key=trial.get)
-    best_mode = int(best.rsplit(" ", 1)[1])
-    tuned = dict(candidates)[best.rsplit(",", 1)[0]]
-    for k, c in enumerate(ctxs):
-        c.set_variants(list(tuned[int(sizes[k])]))
-        c.upload(saved[k])
     B = G.Batch(ctxs)
-    B.set_graph_mode(best_mode)
+    # the batch tunes itself as a whole (a lone instance's choice serves latency;
+    # in the batch the groups share the device): variants per shape group and
+    # the graph mode, timed over 20 batch steps each, states restored after.
+    # Forced choices skip it (its timing replays graphs, which kernel tracers avoid).
+    tune_us = None
+    if force_mode is None and force_variants is None:
+        t_tune = time.perf_counter()
+        tune_us, best_mode = B.autotune(0, 20)
+        t_tune = time.perf_counter() - t_tune
+    else:
+        best_mode = 0 if force_mode is None else force_mode
+        B.set_graph_mode(best_mode)
+        t_tune = 0.0
+    tuned = {}
+    for k, c in enumerate(ctxs):
+        tuned.setdefault(int(sizes[k]), c.variants())
     active = [True] * n_inst
     e_prev = [None] * n_inst
     last_change = [None] * n_inst
@@ -356,7 +341,8 @@ def batched_sweep(G, n_inst=64, seed=4, max_steps=20000, tol=1e-8):
             "grid_point_steps_per_s": round(point_steps / el, 1),
             "effective_GBps": round(bytes_total / el / 1e9, 1),
             "variants_by_size": {int(k): list(v) for k, v in tuned.items()}, "graph_mode": best_mode,
-            "batch_tuning_us_per_batch_step": {k: round(v * 1e6, 1) for k, v in trial.items()},
+            "batch_autotune_us_per_batch_step": round(tune_us, 1) if tune_us is not None else None,
+            "batch_autotune_s": round(t_tune, 2),
             "steps_to_converge": {"min": int(min(steps)), "max": int(max(steps))},
             "convergence_by_omega": by_omega,
             "last_rel_energy_change_per_100_steps": {
@@ -445,6 +431,10 @@ def main():
                     help="profiling: run one workload only (copy: a plain 256 MiB device copy, the "
                          "calibration kernel for the HBM byte counters)")
     ap.add_argument("--steps3", type=int, default=200, help="timed 256^3 steps per precision")
+    ap.add_argument("--batch-steps", type=int, default=20000, help="batched sweep: most steps per instance")
+    ap.add_argument("--batch-mode", type=int, default=None, choices=[0, 1, 2],
+                    help="batched sweep: force the graph mode (2: no graphs, for kernel tracers)")
+    ap.add_argument("--batch-variants", type=str, default=None, help="batched sweep: force the variants x,y,z")
     ap.add_argument("--graph-steps", type=int, default=32,
                     help="steps per captured graph (library default 8; 32 saves ~1.5 %% of graph launch "
                          "gaps here; 0: every sweep launched individually, e.g. under a tracer)")
@@ -596,7 +586,9 @@ def main():
 
     # ---------------- batched sweep and serving stream ----------------
     if not args.no_extra and args.only in (None, "batch"):
-        extra["batched_sweep_64_uneven_fp32"] = batched_sweep(G)
+        extra["batched_sweep_64_uneven_fp32"] = batched_sweep(
+            G, max_steps=args.batch_steps, force_mode=args.batch_mode,
+            force_variants=[int(v) for v in args.batch_variants.split(",")] if args.batch_variants else None)
         log("batched sweep:", json.dumps(extra["batched_sweep_64_uneven_fp32"]))
     if not args.no_extra and args.only in (None, "serve"):
         # a burst far above capacity (throughput, queueing latency), then the

@@ -34,8 +34,8 @@ ABI_SYMBOLS = [
     "gpue_params_default", "gpue_create", "gpue_destroy", "gpue_last_error", "gpue_set_field",
     "gpue_set_field_eqn", "gpue_upload_wfc", "gpue_download_wfc", "gpue_evolve", "gpue_parsum",
     "gpue_energy", "gpue_energy_consistent", "gpue_vortex_marker_2d", "gpue_vortex_marker_2d_noalloc", "gpue_vortex_list_2d", "gpue_get_grid", "gpue_set_g",
-    "gpue_sync", "gpue_set_ramp", "gpue_autotune", "gpue_set_variants", "gpue_set_graph_steps", "gpue_prepare_evolve", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
-    "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_set_graph_mode", "gpue_batch_evolve", "gpue_batch_energy", "gpue_batch_last_error",
+    "gpue_sync", "gpue_set_ramp", "gpue_autotune", "gpue_set_variants", "gpue_get_variants", "gpue_set_graph_steps", "gpue_prepare_evolve", "gpue_set_timing", "gpue_timing_get", "gpue_step_bytes", "gpue_sweep_profile", "gpue_math_eval", "gpue_eqn_parse", "gpue_eqn_eval",
+    "gpue_batch_create", "gpue_batch_set_active", "gpue_batch_set_graph_mode", "gpue_batch_autotune", "gpue_batch_evolve", "gpue_batch_energy", "gpue_batch_last_error",
     "gpue_batch_destroy",
     "gpue_server_create", "gpue_server_submit", "gpue_server_poll", "gpue_server_drain", "gpue_server_clock",
     "gpue_server_destroy",
@@ -124,6 +124,8 @@ def lib():
         L.gpue_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
         L.gpue_batch_set_active.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.gpue_batch_set_graph_mode.argtypes = [vp, ctypes.c_int]
+        L.gpue_get_variants.argtypes = [vp, ip]
+        L.gpue_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip]
         L.gpue_batch_evolve.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.gpue_batch_energy.argtypes = [vp, dp, dp]
         L.gpue_batch_last_error.argtypes = [vp]
@@ -351,6 +353,11 @@ class Context:
         call of num_steps steps."""
         self._check(lib().gpue_set_ramp(self._h, int(ramp_type), int(num_steps)), "set_ramp")
 
+    def variants(self):
+        v = np.zeros(3, dtype=np.int32)
+        self._check(lib().gpue_get_variants(self._h, _ip(v)), "get_variants")
+        return tuple(int(a) for a in v)
+
     def set_variants(self, variants):
         v = np.ascontiguousarray(variants, dtype=np.int32)
         self._check(lib().gpue_set_variants(self._h, _ip(v)), "set_variants")
@@ -398,8 +405,17 @@ class Batch:
             msg = lib().gpue_batch_last_error(self._h)
             raise GpueError(st, f"{what}: {msg.decode() if msg else ''}")
 
+    def autotune(self, gstate=0, steps=20):
+        """Variants per shape group and the graph mode, timed on the whole batch;
+        states are restored.  Returns (best microseconds per batch step, graph mode)."""
+        us = ctypes.c_double()
+        mode = ctypes.c_int()
+        self._check(lib().gpue_batch_autotune(self._h, gstate, steps, ctypes.byref(us), ctypes.byref(mode)),
+                    "batch_autotune")
+        return us.value, mode.value
+
     def set_graph_mode(self, mode):
-        """0: one graph per shape group (default); 1: one graph for the whole batch."""
+        """0: one graph per shape group (default); 1: one graph for the whole batch; 2: no graphs."""
         self._check(lib().gpue_batch_set_graph_mode(self._h, int(mode)), "set_graph_mode")
 
     def set_active(self, index, active):

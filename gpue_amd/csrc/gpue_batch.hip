@@ -17,6 +17,8 @@
 // gpue_batch_create.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -287,7 +289,7 @@ void gpue_batch_destroy(gpue_batch* b) {
 const char* gpue_batch_last_error(const gpue_batch* b) { return b ? b->err.c_str() : "null batch"; }
 
 gpue_status gpue_batch_set_graph_mode(gpue_batch* b, int mode) {
-  if (!b || (mode != 0 && mode != 1)) return GPUE_EINVAL;
+  if (!b || mode < 0 || mode > 2) return GPUE_EINVAL;
   if (mode != b->graph_mode) {
     gpue_status st = sync_groups(b);
     if (st != GPUE_OK) return st;
@@ -451,6 +453,13 @@ gpue_status gpue_batch_evolve(gpue_batch* b, int nsteps, int gstate) {
     BHIP(b, hipEventRecord(b->fork_ev, b->origin));
     for (Group& grp : b->groups) BHIP(b, hipStreamWaitEvent(grp.stream, b->fork_ev, 0));
   }
+  if (b->graph_mode == 2) {   // no graphs: every phase launched on its own (kernel traces)
+    for (; nsteps - s >= 2; s += 2) {
+      gpue_status st = launch_phase(b, gstate, 0);
+      if (st == GPUE_OK) st = launch_phase(b, gstate, 1);
+      if (st != GPUE_OK) return st;
+    }
+  }
   if (nsteps - s >= 2) {
     for (Group& grp : b->groups) {
       if (grp.gexec[gstate]) continue;
@@ -490,6 +499,113 @@ gpue_status gpue_batch_evolve(gpue_batch* b, int nsteps, int gstate) {
     if (gstate == 0) b->ctx[i]->pending_norm = true;
   }
   return GPUE_OK;
+}
+
+// Host time per step of `steps` steps of the whole batch (after two steps that
+// capture any graph the current choice needs).
+static gpue_status time_batch(gpue_batch* b, int gstate, int steps, double* us) {
+  gpue_status st = gpue_batch_evolve(b, 2, gstate);
+  if (st != GPUE_OK) return st;
+  const auto t0 = std::chrono::steady_clock::now();
+  st = gpue_batch_evolve(b, steps, gstate);   // returns after the groups' streams finished
+  if (st != GPUE_OK) return st;
+  *us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / steps;
+  return GPUE_OK;
+}
+
+// Every member of group g takes variant v on axis a; the launch arguments and
+// graphs of the batch follow.
+static gpue_status set_group_variant(gpue_batch* b, Group& g, int a, int v) {
+  for (int idx : g.members) {
+    gpue_ctx* c = b->ctx[idx];
+    int vars[3] = {c->var[0], c->var[1], c->var[2]};
+    vars[a] = v;
+    gpue_status st = gpue_set_variants(c, vars);
+    if (st != GPUE_OK) return bfail(b, st, c->err);
+  }
+  fill_args(b);
+  drop_batch_graphs(b);
+  return upload_args(b);
+}
+
+gpue_status gpue_batch_autotune(gpue_batch* b, int gstate, int steps, double* us_per_step, int* graph_mode) {
+  if (!b || steps < 2 || (gstate != 0 && gstate != 1)) return GPUE_EINVAL;
+  gpue_status st = sync_groups(b);
+  if (st != GPUE_OK) return st;
+  // save every instance's state: psi into its scratch buffer, the step counter,
+  // the pending norm and its value
+  const size_t n = b->ctx.size();
+  std::vector<int> next(n), pend(n), cls = b->cls;
+  std::vector<double> nrm(n);
+  const int phase = b->phase;
+  for (size_t i = 0; i < n; ++i) {
+    gpue_ctx* c = b->ctx[i];
+    next[i] = c->next_step;
+    pend[i] = c->pending_norm;
+    BHIP(b, hipMemcpyAsync(c->tmp, c->psi, c->N * c->esz, hipMemcpyDeviceToDevice, c->stream));
+    BHIP(b, hipMemcpyAsync(&nrm[i], c->results, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BHIP(b, hipStreamSynchronize(c->stream));
+  }
+  double best = 0.0;
+  st = time_batch(b, gstate, steps, &best);
+  // coordinate descent: groups with the most points first, then each axis
+  std::vector<size_t> order(b->groups.size());
+  for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    return b->groups[x].proto->N * (long)b->groups[x].members.size() >
+           b->groups[y].proto->N * (long)b->groups[y].members.size();
+  });
+  for (size_t k = 0; k < order.size() && st == GPUE_OK; ++k) {
+    Group& g = b->groups[order[k]];
+    for (int a = 0; a < g.proto->p.dimnum && st == GPUE_OK; ++a) {
+      const int keep = g.proto->var[a];
+      int pick = keep;
+      // row sweeps: plan and twiddle loading (variants 0-3); strided: plan and tile width (0-5)
+      const int nv = a == 0 ? 4 : N_VARIANTS;
+      for (int v = 0; v < nv && st == GPUE_OK; ++v) {
+        if (v == keep) continue;
+        st = set_group_variant(b, g, a, v);
+        double us = 0.0;
+        if (st == GPUE_OK) st = time_batch(b, gstate, steps, &us);
+        if (st == GPUE_OK && us < best) {
+          best = us;
+          pick = v;
+        }
+      }
+      if (st == GPUE_OK && g.proto->var[a] != pick) st = set_group_variant(b, g, a, pick);
+    }
+  }
+  // then the graph mode: per-group graphs or one graph for the whole batch
+  if (st == GPUE_OK) {
+    const int m0 = b->graph_mode == 2 ? 0 : b->graph_mode, m1 = 1 - m0;
+    st = gpue_batch_set_graph_mode(b, m0);
+    double us0 = best, us1 = 0.0;
+    if (st == GPUE_OK) st = time_batch(b, gstate, steps, &us0);
+    if (st == GPUE_OK) st = gpue_batch_set_graph_mode(b, m1);
+    if (st == GPUE_OK) st = time_batch(b, gstate, steps, &us1);
+    if (st == GPUE_OK && us0 <= us1) st = gpue_batch_set_graph_mode(b, m0);
+    best = std::min(us0, us1);
+  }
+  // restore the states (also after a failure, as far as the device allows)
+  gpue_status rs = sync_groups(b);
+  for (size_t i = 0; i < n; ++i) {
+    gpue_ctx* c = b->ctx[i];
+    if (hipMemcpyAsync(c->psi, c->tmp, c->N * c->esz, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->results, &nrm[i], sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rs = bfail(b, GPUE_EHIP, "restoring an instance after tuning");
+    c->next_step = next[i];
+    c->pending_norm = pend[i];
+  }
+  b->cls = cls;
+  b->phase = phase;
+  fill_args(b);
+  drop_batch_graphs(b);
+  if (rs == GPUE_OK) rs = upload_args(b);
+  if (st == GPUE_OK) st = rs;
+  if (st == GPUE_OK && us_per_step) *us_per_step = best;
+  if (st == GPUE_OK && graph_mode) *graph_mode = b->graph_mode;
+  return st;
 }
 
 gpue_status gpue_batch_energy(gpue_batch* b, double* energy, double* mu) {

@@ -1790,4 +1790,10 @@ gpue_status gpue_set_variants(gpue_ctx* c, const int* variants) {
   return GPUE_OK;
 }
 
+gpue_status gpue_get_variants(const gpue_ctx* c, int* variants) {
+  if (!c || !variants) return GPUE_EINVAL;
+  for (int a = 0; a < 3; ++a) variants[a] = c->var[a];
+  return GPUE_OK;
+}
+
 }  // extern "C"

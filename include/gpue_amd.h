@@ -160,6 +160,8 @@ gpue_status gpue_sync(gpue_ctx *c);
  * 2, 3).  gpue_set_variants forces a choice. */
 gpue_status gpue_autotune(gpue_ctx *c, int reps, int *variants);
 gpue_status gpue_set_variants(gpue_ctx *c, const int *variants);
+/* The kernel variants in use per axis (what gpue_set_variants or an autotuner set). */
+gpue_status gpue_get_variants(const gpue_ctx *c, int *variants);
 /* Steady-state steps are captured as HIP graphs of up to `steps` consecutive
  * steps (even, default 8) and replayed; 0 launches every sweep individually.
  * No reference counterpart (GPUE launches each kernel from the host loop). */
@@ -234,8 +236,18 @@ gpue_status gpue_batch_energy(gpue_batch *b, double *energy, double *mu);
 /* How pairs of steps are replayed: 0 (default) one graph per shape group on the
  * group's stream, launched group by group; 1 one graph for the whole batch, a
  * branch per group forked onto the group streams, so a pair of steps of every
- * instance is one launch. */
+ * instance is one launch; 2 no graphs (every sweep launched on its own, for
+ * kernel tracers). */
 gpue_status gpue_batch_set_graph_mode(gpue_batch *b, int mode);
+/* No counterpart in the reference (GPUE runs one problem per process): choose
+ * the kernel variants of every shape group, and the graph mode, for the batch
+ * as a whole (a lone instance's choice serves latency; in a batch the groups
+ * share the device).  Coordinate descent over the groups, largest first, and
+ * their axes; each candidate is timed over `steps` steps of gstate of the whole
+ * batch.  Every instance's psi, step counter and pending norm are restored
+ * afterwards.  us_per_step, graph_mode (either may be NULL): the best time per
+ * batch step and the graph mode chosen. */
+gpue_status gpue_batch_autotune(gpue_batch *b, int gstate, int steps, double *us_per_step, int *graph_mode);
 const char *gpue_batch_last_error(const gpue_batch *b);
 void        gpue_batch_destroy(gpue_batch *b);
 

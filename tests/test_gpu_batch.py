@@ -21,7 +21,7 @@ def make(n, g, om, prec=64, seed=1, gamma=1.0):
     return c, kw, psi
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_batch_equals_alone(prec, mode):
     specs = [(64, 100.0, 0.3), (128, 300.0, 0.7), (64, 150.0, 0.5), (192, 500.0, 0.9), (128, 50.0, 0.0)]
@@ -38,7 +38,7 @@ def test_batch_equals_alone(prec, mode):
     batch_ctx[2].evolve(1, 0)
     alone[2].evolve(1, 0)
     b = G.Batch(batch_ctx)
-    b.set_graph_mode(mode)   # 1: one graph for the whole batch (a branch per shape group)
+    b.set_graph_mode(mode)   # 1: one graph for the whole batch (a branch per shape group); 2: no graphs
     b.evolve(10, 0)
     b.evolve(7, 0)
     b.set_active(4, False)
@@ -69,4 +69,37 @@ def test_batch_energy_equals_per_context():
     e, m = b.energy()
     for k, c in enumerate(ctxs):
         assert (e[k], m[k]) == c.energy(), k
+    b.close()
+
+
+def test_batch_autotune_restores_state_and_keeps_parity():
+    # the batch's own tuner times variants per shape group and the graph mode on
+    # the instances themselves, then restores psi, step counters and pending
+    # norms; the tuned batch then advances within the FP64 bar of lone contexts
+    specs = [(64, 100.0, 0.3), (128, 300.0, 0.7), (64, 150.0, 0.5), (128, 50.0, 0.0)]
+    ctxs, alone = [], []
+    for k, (n, g, om) in enumerate(specs):
+        c, kw, psi = make(n, g, om, 64, seed=k + 7)
+        ctxs.append(c)
+        a = G.Context(**kw)
+        a.upload(psi)
+        a.parsum()
+        a.set_graph_steps(0)
+        alone.append(a)
+    b = G.Batch(ctxs)
+    b.evolve(3, 0)   # a pending norm and an odd step count before tuning
+    for a in alone:
+        a.evolve(3, 0)
+    before = [c.download() for c in ctxs]
+    us, mode = b.autotune(0, 4)
+    assert us > 0 and mode in (0, 1)
+    for c, p in zip(ctxs, before):
+        assert np.array_equal(c.download(), p)
+    b.evolve(8, 0)
+    b.evolve(4, 1)
+    for k, (c, a) in enumerate(zip(ctxs, alone)):
+        a.evolve(8, 0, first_step=3)
+        a.evolve(4, 1, first_step=11)
+        x, y = c.download(), a.download()
+        assert np.linalg.norm(x - y) <= 1e-10 * np.linalg.norm(y), k
     b.close()
