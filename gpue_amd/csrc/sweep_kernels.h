@@ -103,7 +103,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
     if (flags & F_HALFV_POST) halfV_all<T, E, X, EQ>(p, v, pos0);
     if (flags & F_GAUGE_POST) {
       fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
-      gauge_all<T, E, 0, X, EQ>(p, v, posL, a.invN);
+      if (!gauge_rot_lin<T, N, TPL, RFL, E, 0, X, EQ>(p, v, tid, yi, a.invN)) gauge_all<T, E, 0, X, EQ>(p, v, posL, a.invN);
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     }
   } else {
@@ -132,7 +132,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
     }
     fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
     if constexpr (MODE == RM_GAUGE || MODE == RM_GAUGE_FWD) {
-      gauge_all<T, E, 0, X, EQ>(p, v, posL, a.invN);
+      if (!gauge_rot_lin<T, N, TPL, RFL, E, 0, X, EQ>(p, v, tid, yi, a.invN)) gauge_all<T, E, 0, X, EQ>(p, v, posL, a.invN);
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     } else if constexpr (MODE == RM_K) {
       kstep_all<T, E, X, EQ>(p, v, posL, a.invN);
@@ -143,7 +143,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
       if (flags & F_HALFV_POST) halfV_all<T, E, X, EQ>(p, v, pos0);
       if (flags & F_GAUGE_POST) {
         fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
-        gauge_all<T, E, 0, X, EQ>(p, v, posL, a.invN);
+        if (!gauge_rot_lin<T, N, TPL, RFL, E, 0, X, EQ>(p, v, tid, yi, a.invN)) gauge_all<T, E, 0, X, EQ>(p, v, posL, a.invN);
         fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
       }
     } else if constexpr (MODE == RM_KSUM) {
@@ -238,8 +238,11 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
     return a.axis == 1 ? Pt{xq, e, o} : Pt{xq, o, e};
   };
   auto gauge_here = [&](cx<T>* w) {
-    if (a.axis == 1) gauge_all<T, E, 1, X, EQ>(p, w, posL, a.invN);
-    else gauge_all<T, E, 2, X, EQ>(p, w, posL, a.invN);
+    if (a.axis == 1) {
+      if (!gauge_rot_lin<T, N, TPL, RFL, E, 1, X, EQ>(p, w, tid, xq, a.invN)) gauge_all<T, E, 1, X, EQ>(p, w, posL, a.invN);
+    } else {
+      gauge_all<T, E, 2, X, EQ>(p, w, posL, a.invN);
+    }
   };
 
   if constexpr (MODE == CM_INV || MODE == CM_INV_VG) {

@@ -559,6 +559,80 @@ __device__ __forceinline__ void gauge_all(const Phys& p, cx<T>* v, F pos, double
   apply_phase_all<T, E, X>(v, arg, p.gstate, invn);
 }
 
+// The rotation gauge on a line along AXIS (x: A_x = -y omega omegaX; y:
+// A_y = x omega omegaY; both constant along the line) multiplies the point of
+// signed wave-number index m by exp(-a m) (imaginary time) or exp(-i a m) (real
+// time), a = A dk Dt, dk = k[1].  In the boundary layout of radix R a lane holds
+// the points tid + TPL q + (N/R) s (r = q R + s), less N from the Nyquist index
+// on, so the E factors are products of four per-lane factors and their powers:
+// exp(-a tid), exp(-a N/R), exp(-a TPL) and exp(a N), i.e. three or four
+// exponentials per lane instead of E (each product adds a rounding: a few ulp).
+// FP64 kernels only; false (nothing applied) for other gauges, for FP32, or when
+// |a| N is large enough for exp(a N) to leave the double range.
+template <typename T, int N, int TPL, int R, int E, int AXIS, bool X, bool EQ>
+__device__ __forceinline__ bool gauge_rot_lin(const Phys& p, cx<T>* v, int tid, int other, double invn) {
+  if constexpr (sizeof(T) != 8 || X || AXIS == 2) {
+    return false;
+  } else {
+    if (p.a_kind != A_ROTATION) return false;
+    const double A = AXIS == 0 ? -p.yg[other] * p.omega * p.omegaX : p.xg[other] * p.omega * p.omegaY;
+    const double dk = AXIS == 0 ? p.pxg[1] : p.pyg[1];
+    const double a = A * dk * p.Dt;
+    if (!(fabs(a) * N <= 600.0)) return false;
+    constexpr int Q = E / R, NR = N / R;
+    if (p.gstate == 0) {
+      double w[R];
+      w[0] = WMath<double>::ex(-a * tid) * invn;
+      const double W = WMath<double>::ex(-a * NR), Z = WMath<double>::ex(a * N);
+      const double U = Q > 1 ? WMath<double>::ex(-a * TPL) : 1.0;
+#pragma unroll
+      for (int k = 1; k < R; ++k) w[k] = w[k - 1] * W;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const int r = q * R + k;
+          const double f = (tid + TPL * q + NR * k >= N / 2) ? w[k] * Z : w[k];
+          v[r] = cscale(v[r], f);
+        }
+        if (q + 1 < Q) {
+#pragma unroll
+          for (int k = 0; k < R; ++k) w[k] *= U;
+        }
+      }
+    } else {
+      // exp(-i x) = (cos x, -sin x)
+      auto ph = [](double x) {
+        double sn, cs;
+        WMath<double>::sc(x, &sn, &cs);
+        return mk(cs, -sn);
+      };
+      cx<double> w[R];
+      const cx<double> b = ph(a * tid);
+      w[0] = mk(b.x * invn, b.y * invn);
+      const cx<double> W = ph(a * NR), Zc = ph(a * N);
+      const cx<double> Z = mk(Zc.x, -Zc.y);   // exp(+i a N)
+      const cx<double> U = Q > 1 ? ph(a * TPL) : mk(1.0, 0.0);
+#pragma unroll
+      for (int k = 1; k < R; ++k) w[k] = cmul(w[k - 1], W);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const int r = q * R + k;
+          const cx<double> f = (tid + TPL * q + NR * k >= N / 2) ? cmul(w[k], Z) : w[k];
+          v[r] = cmul(f, v[r]);
+        }
+        if (q + 1 < Q) {
+#pragma unroll
+          for (int k = 0; k < R; ++k) w[k] = cmul(w[k], U);
+        }
+      }
+    }
+    return true;
+  }
+}
+
 // Per-axis K factor table in the working precision.
 template <typename T>
 struct KTab {
