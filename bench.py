@@ -85,24 +85,25 @@ def run_schedule(ctx, imag, real):
 # (each scaled by its reading for a plain 256 MiB device copy: 2.0 and 1.0), per
 # kernel instance and workload; the passes and tables are in profiles/.
 MEASURED_TRAFFIC = {
-    "gpue::col_sweep<double, 512, 5, 2>@512sq": 8813880,
-    "gpue::col_sweep<double, 512, 5, 3>@512sq": 8791408,
-    "gpue::col_sweep<double, 512, 5, 5>@512sq": 8711989,
-    "gpue::col_sweep<double, 512, 5, 6>@512sq": 8977974,
-    "gpue::row_sweep<double, 512, 3, 0>@512sq": 8699992,
-    "gpue::row_sweep<double, 512, 3, 1>@512sq": 8989775,
-    "gpue::row_sweep<double, 512, 3, 2>@512sq": 8799669,
-    "gpue::row_sweep<double, 512, 3, 7>@512sq": 8833750,
-    "gpue::col_sweep<double, 256, 0, 0>@256cube": 537014011,
-    "gpue::col_sweep<double, 256, 0, 1>@256cube": 537005628,
-    "gpue::col_sweep<double, 256, 2, 2>@256cube": 537390597,
-    "gpue::col_sweep<float, 256, 0, 0>@256cube": 268557347,
-    "gpue::col_sweep<float, 256, 0, 1>@256cube": 268550919,
-    "gpue::col_sweep<float, 256, 0, 2>@256cube": 268820343,
-    "gpue::row_sweep<double, 256, 1, 0>@256cube": 537112246,
-    "gpue::row_sweep<double, 256, 1, 1>@256cube": 538254835,
-    "gpue::row_sweep<float, 256, 0, 0>@256cube": 268637783,
-    "gpue::row_sweep<float, 256, 0, 1>@256cube": 268944100,
+    "gpue::col_sweep<double, 512, 5, 2>@512sq": 8816200,
+    "gpue::col_sweep<double, 512, 5, 3>@512sq": 8792219,
+    "gpue::col_sweep<double, 512, 5, 5>@512sq": 8701790,
+    "gpue::col_sweep<double, 512, 5, 6>@512sq": 8970078,
+    "gpue::row_sweep<double, 512, 3, 0>@512sq": 8716621,
+    "gpue::row_sweep<double, 512, 3, 1>@512sq": 8970702,
+    "gpue::row_sweep<double, 512, 3, 2>@512sq": 8777439,
+    "gpue::row_sweep<double, 512, 3, 7>@512sq": 8827463,
+    "gpue::col_sweep<double, 256, 0, 0>@256cube": 537018167,
+    "gpue::col_sweep<double, 256, 0, 1>@256cube": 537011448,
+    "gpue::col_sweep<double, 256, 0, 2>@256cube": 537413787,
+    "gpue::col_sweep<double, 256, 2, 2>@256cube": 537373553,
+    "gpue::col_sweep<float, 256, 0, 0>@256cube": 268573608,
+    "gpue::col_sweep<float, 256, 0, 1>@256cube": 268553861,
+    "gpue::col_sweep<float, 256, 0, 2>@256cube": 268807542,
+    "gpue::row_sweep<double, 256, 1, 0>@256cube": 537123456,
+    "gpue::row_sweep<double, 256, 1, 1>@256cube": 538228046,
+    "gpue::row_sweep<float, 256, 0, 0>@256cube": 268649093,
+    "gpue::row_sweep<float, 256, 0, 1>@256cube": 268942663,
 }
 
 RM = {"row_fwd": 0, "row_inv": 1, "row_gauge": 2, "row_full": 3, "row_ksum": 4, "row_inv_fwd": 5,
