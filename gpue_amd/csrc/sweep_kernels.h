@@ -44,6 +44,7 @@ struct ColCfg {
   static constexpr int TC = (TC0 >> NARROW) > 0 ? (TC0 >> NARROW) : 1;
   static constexpr int G = TC0 / TC;   // tiles per default-width tile
   static constexpr int WG = TPL * TC;
+  static_assert(TC0 >= 1 && G >= 1 && WG <= 1024, "strided tile does not fit a workgroup");
 };
 
 // scalarDiv_wfcNorm (src/kernels.cu:479-486) divides by sqrt(sum * dV); the
