@@ -98,3 +98,26 @@ def test_cli_rejects_flags_outside_the_step(tmp_path, args, message):
     r = subprocess.run([GPUE_BIN, "-d", str(tmp_path)] + args, capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert message in r.stderr
+
+
+def test_sweep_units_match_their_generator():
+    # one translation unit per (precision, axis length, plan), as gen_units.py writes them
+    import importlib.util
+    import os
+
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpue_amd", "csrc")
+    spec = importlib.util.spec_from_file_location("gen_units", os.path.join(here, "gen_units.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    units = sorted(f for f in os.listdir(here) if f.startswith("sweep_f") and f.endswith(".hip"))
+    want = sorted(f"sweep_{p}_{n}_v{v}.hip" for n in gen.SIZES for p in ("f64", "f32") for v in (0, 1))
+    assert units == want
+    for f in units:
+        prec, n, v = f[:-4].split("_")[1:]
+        T = "double" if prec == "f64" else "float"
+        text = open(os.path.join(here, f)).read()
+        assert f"GPUE_SWEEP_TU({T}, {n}, {v[1:]}, {f[:-4]})" in text, f
+    # every generated length is a supported grid size of the library (the dispatch lists the same)
+    src = open(os.path.join(here, "sweep_dispatch.hip")).read()
+    for n in gen.SIZES:
+        assert f"GPUE_CASE({n})" in src and f"case {n}:" in src, n
