@@ -10,6 +10,10 @@ Every expected value here is computed independently of the oracle:
                         gauge operators (src/evolution.cu:163-257, 727-870) and
                         V += 1/2 m |A|^2 (src/operators.cu:1006-1007), imaginary and real time
 * gpue_gauge_3d.npz     the same in 3D (src/evolution.cu:5-160)
+* gpue_gauge_box.npz    the same on a 1D line (only the x operator; the y grid of an
+                        unused axis is 0, so A_x = 0 and V gains 1/2 (x omega)^2 from A_y),
+                        a rectangular 2D box (n_x != n_y, xMax != yMax, gammaY != 1) and a
+                        3D box (n_x != n_y = n_z, three different extents)
 * vortex_cases.npz      phase fields with vortices imprinted inside known plaquettes and
                         the markers GPUE's skip/sign/mask rules give for them, derived
                         from the imprint geometry (src/tracker.cu:102-161)
@@ -141,6 +145,74 @@ def gpue_gauge(dim):
                         steps=steps, gdt=gdt, dt=dt, psi0=psi0, psi_imag=imag, psi_real=w)
 
 
+def gpue_gauge_box():
+    """GPUE's step with the per-axis gauge operators on grids whose axes differ
+    (the generalisation of src/evolution.cu:163-257, 727-870 to rectangular
+    boxes) and on a 1D line, imaginary then real time, in numpy."""
+    cases = {
+        "line": dict(dims=[128], maxes=[8.0], g=10.0, omega=0.3, gammaY=1.0, oz=0.0, steps=10, seed=21),
+        "rect": dict(dims=[64, 32], maxes=[8.0, 5.0], g=40.0, omega=0.45, gammaY=1.2, oz=0.0, steps=10, seed=22),
+        "box": dict(dims=[32, 16, 16], maxes=[6.0, 4.0, 5.0], g=30.0, omega=0.4, gammaY=0.9, oz=1.5, steps=6,
+                    seed=23),
+    }
+    gdt, dt = 1e-3, 5e-4
+    out = {}
+    for name, c in cases.items():
+        dim = len(c["dims"])
+        ax = [gpue_grid(n, m) for n, m in zip(c["dims"], c["maxes"])]
+        while len(ax) < 3:
+            ax.append((np.zeros(1), np.zeros(1), 1.0))   # an unused axis: one point at 0
+        (x, kx, dx), (y, ky, dy), (z, kz, dz) = ax
+        # numpy axes (z, y, x); length-1 axes for the unused ones
+        Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+        KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+        om = c["omega"]
+        Ax, Ay = -Y * om, X * om
+        V = 0.5 * (X ** 2 + (c["gammaY"] * Y) ** 2 + (c["oz"] * Z) ** 2 + Ax ** 2 + Ay ** 2)
+        K = 0.5 * (KX ** 2 + KY ** 2 + KZ ** 2)
+        dV = dx * (dy if dim > 1 else 1.0) * (dz if dim > 2 else 1.0)
+        rng = np.random.default_rng(c["seed"])
+        psi0 = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 2) * (1 + 0.01 * rng.uniform(-1, 1, X.shape))
+        psi0 = psi0 * np.exp(1j * rng.uniform(0, 2 * np.pi, X.shape))
+        psi0 = psi0.astype(np.complex128)
+        psi0 /= np.sqrt(np.sum(np.abs(psi0) ** 2) * dV)
+        g = c["g"]
+
+        def step(w, i, gstate):
+            D = gdt if gstate == 0 else dt
+            op = (lambda a: np.exp(-a)) if gstate == 0 else (lambda a: np.exp(-1j * a))
+            half = lambda w: op(V * D / 2 + g * np.abs(w) ** 2 * (0.5 * D)) * w
+            w = half(w)
+            w = np.fft.ifftn(op(K * D) * np.fft.fftn(w))
+            w = half(w)
+            gx = lambda w: np.fft.ifft(op(Ax * KX * D) * np.fft.fft(w, axis=2), axis=2)
+            gy = lambda w: np.fft.ifft(op(Ay * KY * D) * np.fft.fft(w, axis=1), axis=1)
+            if dim == 1:
+                w = gx(w)
+            elif i % 2 == 1:
+                w = gy(gx(w))          # (the z operator is the identity: A_z = 0)
+            else:
+                w = gx(gy(w))
+            if gstate == 0:
+                w = w / np.sqrt(np.sum(np.abs(w) ** 2) * dV)
+            return w
+
+        w = psi0.copy()
+        for i in range(c["steps"]):
+            w = step(w, i, 0)
+        imag = w.copy()
+        for i in range(c["steps"]):
+            w = step(w, i, 1)
+        shape = tuple(c["dims"][::-1])
+        out[f"{name}_dims"] = np.array(c["dims"])
+        out[f"{name}_maxes"] = np.array(c["maxes"])
+        out[f"{name}_par"] = np.array([g, om, c["gammaY"], c["oz"], c["steps"], gdt, dt])
+        out[f"{name}_psi0"] = psi0.reshape(shape)
+        out[f"{name}_imag"] = imag.reshape(shape)
+        out[f"{name}_real"] = w.reshape(shape)
+    np.savez_compressed(os.path.join(HERE, "gpue_gauge_box.npz"), **out)
+
+
 def vortex_cases():
     """Unit-modulus fields with vortices centred inside chosen plaquettes.
 
@@ -193,5 +265,6 @@ if __name__ == "__main__":
     gpe2d_matlab()
     gpue_gauge(2)
     gpue_gauge(3)
+    gpue_gauge_box()
     vortex_cases()
     print("golden fixtures written to", HERE)

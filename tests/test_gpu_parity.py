@@ -425,3 +425,24 @@ def test_parameter_change_right_after_graph_replays(change):
         out.append(c.download())
         c.close()
     assert rel_l2(out[0], out[1]) <= 1e-13
+
+
+@pytest.mark.parametrize("case", ["line", "rect", "box"])
+def test_gauge_box_fixture(golden, case):
+    # the HIP path against the numpy transcription of GPUE's gauge step on a 1D line and on
+    # boxes whose axes differ in length and extent (tests/golden/make_golden.py)
+    g = golden("gpue_gauge_box.npz")
+    dims = [int(v) for v in g[f"{case}_dims"]]
+    maxes = [float(v) for v in g[f"{case}_maxes"]]
+    gg, om, gam, oz, steps, gdt, dt = [float(v) for v in g[f"{case}_par"]]
+    kw = dict(dimnum=len(dims), g=gg, gpe=1, corotating=1, omega=om, gammaY=gam, omegaZ=oz, gdt=gdt, dt=dt)
+    for a, (n, m) in enumerate(zip(dims, maxes)):
+        kw["xyz"[a] + "Dim"] = n
+        kw["xyz"[a] + "Max"] = m
+    ctx = G.Context(**kw)
+    ctx.upload(g[f"{case}_psi0"])
+    ctx.evolve(int(steps), 0)
+    assert rel_l2(ctx.download(), g[f"{case}_imag"]) <= TOL64
+    ctx.evolve(int(steps), 1)
+    assert rel_l2(ctx.download(), g[f"{case}_real"]) <= TOL64
+    ctx.close()

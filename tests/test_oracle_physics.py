@@ -274,3 +274,50 @@ def test_energy_consistent_is_conserved_in_real_time():
     g1, _ = o.energy()
     assert abs(e1 - e0) < 1e-4 * abs(e0)
     assert abs(g1 - g0) > 100 * abs(e1 - e0)
+
+
+@pytest.mark.parametrize("case", ["line", "rect", "box"])
+def test_gauge_box_crosscheck(golden, case):
+    # GPUE's step with per-axis gauge operators on a 1D line and on boxes whose axes differ,
+    # transcribed in numpy (tests/golden/make_golden.py, gpue_gauge_box)
+    g = golden("gpue_gauge_box.npz")
+    dims = [int(v) for v in g[f"{case}_dims"]]
+    maxes = [float(v) for v in g[f"{case}_maxes"]]
+    gg, om, gam, oz, steps, gdt, dt = [float(v) for v in g[f"{case}_par"]]
+    kw = dict(dimnum=len(dims), g=gg, gpe=1, corotating=1, omega=om, gammaY=gam, omegaZ=oz, gdt=gdt, dt=dt)
+    for a, (n, m) in enumerate(zip(dims, maxes)):
+        kw["xyz"[a] + "Dim"] = n
+        kw["xyz"[a] + "Max"] = m
+    o = O.Oracle(**kw)
+    o.set_wfc(g[f"{case}_psi0"])
+    for i in range(int(steps)):
+        o.step(i, 0)
+    assert rel_l2(o.wfc(), g[f"{case}_imag"]) < 1e-12
+    for i in range(int(steps)):
+        o.step(i, 1)
+    assert rel_l2(o.wfc(), g[f"{case}_real"]) < 1e-12
+
+
+def test_energy_and_mu_weights_by_parseval():
+    # E = <K> + <V> + g/2 int|psi|^4 (energy_calc, src/split_op.cu:208-375) and the chemical
+    # potential mu = <psi|H_GP|psi> = <K> + <V> + g int|psi|^4, both computed here with
+    # numpy.fft on a random state: pins the interaction weights g/2 and g and the kinetic
+    # normalisation independently of the oracle's own sums
+    n, L, g, hbar, mass = 64, 8.0, 137.0, 1.1, 0.9
+    o = O.Oracle(dimnum=2, xDim=n, yDim=n, xMax=L, yMax=L, g=g, gpe=1, hbar=hbar, mass=mass)
+    x = o.grid(0)
+    dx = x[1] - x[0]
+    rng = np.random.default_rng(5)
+    psi = (np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / 3) * (1 + 0.3 * rng.uniform(-1, 1, (n, n)))
+           * np.exp(1j * rng.uniform(0, 2 * np.pi, (n, n))))
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * dx * dx)
+    o.set_wfc(psi)
+    k = 2 * np.pi * np.fft.fftfreq(n, d=dx)
+    K = hbar ** 2 * (k[None, :] ** 2 + k[:, None] ** 2) / (2 * mass)
+    ekin = np.sum(K * np.abs(np.fft.fft2(psi)) ** 2) / (n * n) * dx * dx
+    V = 0.5 * mass * (x[None, :] ** 2 + x[:, None] ** 2)
+    epot = np.sum(V * np.abs(psi) ** 2) * dx * dx
+    e4 = np.sum(np.abs(psi) ** 4) * dx * dx
+    e, mu = o.energy()
+    assert abs(e - (ekin + epot + 0.5 * g * e4)) < 1e-11 * abs(e)
+    assert abs(mu - (ekin + epot + g * e4)) < 1e-11 * abs(mu)
