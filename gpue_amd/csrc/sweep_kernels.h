@@ -27,9 +27,12 @@ struct RowCfg {
 // to a half or a quarter of that width. Small grids have few tiles (64 of
 // them for 512 FP64 columns), so narrower tiles put more CUs to work; the
 // tiles that share a 128-byte line then run on one XCD (col_sweep), so the
-// line is fetched into one L2 only.  Narrowed tiles also load each
+// line is fetched into one L2 only.  Narrowed FP64 tiles also load each
 // transform's twiddles before its first pass (these launches are latency
-// bound and have registers to spare).
+// bound and have registers to spare); FP32 tiles keep per-pass loads, since
+// the up-front tables (hi + lo parts) cost 60-100 more VGPRs and FP32 tiles
+// serve batches whose sweeps are limited by occupancy (82 VGPRs: 5 waves per
+// SIMD; 141-180 with the tables: 2-3).
 template <typename T, int N, int CV>
 struct ColCfg {
   static constexpr int PV = CV & 1, NARROW = CV >> 1;
@@ -201,7 +204,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
   constexpr bool X = (MODE_ & 16) != 0;
   constexpr bool EQ = (MODE_ & 32) != 0;
   constexpr int V = CV & 1;
-  constexpr bool PRE = (CV >> 1) != 0;
+  constexpr bool PRE = (CV >> 1) != 0 && sizeof(T) == 8;
   using P = Plan<N, V>;
   constexpr int TC = ColCfg<T, N, CV>::TC, E = P::E, TPL = P::TPL;
   __shared__ cx<T> lds[P::NP > 1 ? TC * N : 1];
