@@ -591,7 +591,7 @@ def main():
             G, max_steps=args.batch_steps, force_mode=args.batch_mode,
             force_variants=[int(v) for v in args.batch_variants.split(",")] if args.batch_variants else None)
         log("batched sweep:", json.dumps(extra["batched_sweep_64_uneven_fp32"]))
-    if not args.no_extra and args.only in (None, "serve"):
+    if not args.no_extra and args.only == "serve":
         # a burst far above capacity (throughput, queueing latency), then the
         # same mix offered at about half the measured capacity (service latency)
         extra["serving_stream"] = serving_stream(G)
@@ -599,6 +599,21 @@ def main():
         light = round(0.5 * extra["serving_stream"]["throughput_req_s"], 1)
         extra["serving_stream_half_load"] = serving_stream(G, rate=light)
         log("serving stream at half load:", json.dumps(extra["serving_stream_half_load"]))
+    elif not args.no_extra and args.only is None:
+        # the serving stream runs in a process of its own, as a solve service would:
+        # in this process, after the sections above have created and destroyed some
+        # seventy streams, its small requests ran several times slower
+        import subprocess
+
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--only", "serve", "--no-cpu"],
+                           capture_output=True, text=True, timeout=900)
+        sys.stderr.write(r.stderr)
+        if r.returncode != 0:
+            raise RuntimeError(f"serving stream process failed ({r.returncode})")
+        sub = json.loads(r.stdout.strip().splitlines()[-1])["extra"]
+        extra["serving_stream"] = sub["serving_stream"]
+        extra["serving_stream_half_load"] = sub["serving_stream_half_load"]
+        extra["serving_stream"]["process"] = "own"
 
     cpu = None
     if not args.no_cpu and args.only is None:
