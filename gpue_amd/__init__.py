@@ -172,11 +172,13 @@ def make_params(**kw):
 
 
 def math_eval(fn, x, device=0):
-    """Device exp / sin / cos of the fused operators: "exp", "sin", "cos" (FP64)
-    and "sin32", "cos32" (FP32, input rounded to float)."""
+    """Device exp / sin / cos of the fused operators: "exp", "sin", "cos" (FP64),
+    "sin32", "cos32" (FP32, input rounded to float) and the short forms for
+    |x| <= 1/32, "exp_small", "sin_small", "cos_small"."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.empty_like(x)
-    st = lib().gpue_math_eval(device, {"exp": 0, "sin": 1, "cos": 2, "sin32": 3, "cos32": 4}[fn], _dp(x), _dp(y),
+    st = lib().gpue_math_eval(device, {"exp": 0, "sin": 1, "cos": 2, "sin32": 3, "cos32": 4, "exp_small": 5, "sin_small": 6,
+                                          "cos_small": 7}[fn], _dp(x), _dp(y),
                               x.size)
     if st != GPUE_OK:
         raise GpueError(st, "gpue_math_eval failed")

@@ -1702,7 +1702,7 @@ gpue_status gpue_sweep_profile(gpue_ctx* c, int gstate, int reps, int max_kinds,
 }
 
 gpue_status gpue_math_eval(int device, int fn, const double* x, double* y, long n) {
-  if (!x || !y || n < 0 || fn < 0 || fn > 4) return GPUE_EINVAL;
+  if (!x || !y || n < 0 || fn < 0 || fn > 7) return GPUE_EINVAL;
   if (n == 0) return GPUE_OK;
   if (hipSetDevice(device) != hipSuccess) return GPUE_EHIP;
   double* d = nullptr;

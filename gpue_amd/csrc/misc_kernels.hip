@@ -297,9 +297,14 @@ __global__ void k_math_eval(int fn, const double* __restrict__ x, double* __rest
     } else if (fn <= 2) {
       WMath<double>::sc(x[i], &s, &c);
       y[i] = fn == 1 ? s : c;
-    } else {
+    } else if (fn <= 4) {
       WMath<float>::sc((float)x[i], &sf, &cf);
       y[i] = fn == 3 ? sf : cf;
+    } else if (fn == 5) {
+      y[i] = exp_small(x[i]);   // the short forms, for |x| <= SMALL_ARG
+    } else {
+      sincos_small(x[i], &s, &c);
+      y[i] = fn == 6 ? s : c;
     }
   }
 }

@@ -190,6 +190,28 @@ __device__ __forceinline__ double exp_cw(double x) {
   return ldexp(p, (int)k);
 }
 
+// |x| <= 1/32: Taylor polynomials without a reduction (the V/2 phases of
+// real time and most imaginary-time exponents are this small); the first
+// omitted terms, x^9/9! and r^9/9!, r^10/10!, are below 1e-18 relative.
+constexpr double SMALL_ARG = 0.03125;
+__device__ __forceinline__ double exp_small(double x) {
+  using namespace fm;
+  double p = F8;
+  p = fma(p, x, F7); p = fma(p, x, F6); p = fma(p, x, F5); p = fma(p, x, F4); p = fma(p, x, F3);
+  p = fma(p, x, F2); p = fma(p, x, 1.0);
+  return fma(p, x, 1.0);
+}
+__device__ __forceinline__ void sincos_small(double r, double* sn_out, double* cs_out) {
+  using namespace fm;
+  const double z = r * r;
+  double ps = -F7;
+  ps = fma(ps, z, F5); ps = fma(ps, z, -F3);
+  *sn_out = fma(r * z, ps, r);            // r - r^3/3! + r^5/5! - r^7/7!
+  double pc = F8;
+  pc = fma(pc, z, -F6); pc = fma(pc, z, F4); pc = fma(pc, z, -F2);
+  *cs_out = fma(z, pc, 1.0);              // 1 - r^2/2! + ... + r^8/8!
+}
+
 // sin and cos of |r| <= pi/4 (no reduction)
 __device__ __forceinline__ void sincos_poly(double r, double* sn_out, double* cs_out) {
   using namespace fm;
@@ -405,7 +427,21 @@ __device__ __forceinline__ double V_eq(const Phys& p, Pt q, long gid) {
 template <typename T, int E, bool X = false>
 __device__ __forceinline__ void apply_phase_all(cx<T>* v, const T (&arg)[E], int gstate, double s_) {
   const T s = (T)s_;
+  // FP64: a lane whose arguments are all small takes the short polynomials
+  bool tiny = false;
+  if constexpr (sizeof(T) == 8) {
+    tiny = true;
+#pragma unroll
+    for (int r = 0; r < E; ++r) tiny = tiny && fabs(arg[r]) <= SMALL_ARG;
+  }
   if (gstate == 0) {
+    if constexpr (sizeof(T) == 8) {
+      if (tiny) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = cscale(v[r], exp_small(-arg[r]) * s);
+        return;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const T f = WMath<T>::ex(-arg[r]) * s;
@@ -446,6 +482,15 @@ __device__ __forceinline__ void apply_phase_all(cx<T>* v, const T (&arg)[E], int
       fast = fast && WMath<T>::sc_ok(-arg[r]);
     }
     T sn[E], cs[E];
+    if constexpr (sizeof(T) == 8) {
+      if (tiny) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) sincos_small(-arg[r], &sn[r], &cs[r]);
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = cmul(mk(cs[r] * s, sn[r] * s), v[r]);
+        return;
+      }
+    }
     if (small) {
 #pragma unroll
       for (int r = 0; r < E; ++r) WMath<T>::sc_small(-arg[r], &sn[r], &cs[r]);

@@ -195,7 +195,8 @@ gpue_status gpue_step_bytes(gpue_ctx *c, int gstate, double *bytes_per_step, dou
 gpue_status gpue_sweep_profile(gpue_ctx *c, int gstate, int reps, int max_kinds, int *n_kinds,
                                const char **names, double *us_per_launch, double *launches_per_step);
 /* The exp (fn 0), sin (1) and cos (2) the FP64 fused operators evaluate on the
- * device, and the sin (3) / cos (4) of the FP32 ones (input rounded to float),
+ * device, the sin (3) / cos (4) of the FP32 ones (input rounded to float), and
+ * the short small-argument forms exp (5), sin (6), cos (7) used for |x| <= 1/32,
  * applied to n host values (verification of the device math; no reference
  * counterpart). */
 gpue_status gpue_math_eval(int device, int fn, const double *x, double *y, long n);

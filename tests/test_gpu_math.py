@@ -48,3 +48,17 @@ def test_sincos_fp32_matches_libm(fn):
     y = G.math_eval(fn + "32", x)
     assert np.max(np.abs(y - getattr(np, fn)(x))) <= 3e-7
     assert np.isnan(G.math_eval(fn + "32", np.array([np.nan, np.inf]))).all()
+
+
+@pytest.mark.parametrize("fn", ["exp", "sin", "cos"])
+def test_small_argument_forms_match_libm(fn):
+    # the short polynomials the FP64 phase factors take when every argument of a lane
+    # is within 1/32 (no reduction; truncation below 1e-18 relative)
+    rng = np.random.default_rng(6)
+    x = np.concatenate([rng.uniform(-1 / 32, 1 / 32, 200000), np.array([0.0, -0.0, 1 / 32, -1 / 32, 1e-300])])
+    y = G.math_eval(fn + "_small", x)
+    ref = getattr(np, fn)(x)
+    if fn == "exp":
+        assert np.max(np.abs(y - ref) / ref) <= 4e-16
+    else:
+        assert np.max(np.abs(y - ref)) <= 2e-16
