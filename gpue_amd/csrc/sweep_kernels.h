@@ -220,8 +220,11 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
   const long es = (a.axis == 1) ? (long)p.xDim : (long)p.xDim * p.yDim;
   const long base = (a.axis == 1) ? (long)(active ? xi : 0) + (long)o * p.xDim * p.yDim
                                   : (long)(active ? xi : 0) + (long)o * p.xDim;
-  const cx<T>* __restrict__ in = (const cx<T>*)a.in + base;
-  cx<T>* __restrict__ out = (cx<T>*)a.out + base;
+  // a lane's points lie at (tid + bpos(0, r)) * es from the tile's base:
+  // one per-lane pointer, then per register a uniform offset bpos(0, r) * es
+  // (scalar arithmetic) instead of a 64-bit multiply per access
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in + base + (long)tid * es;
+  cx<T>* __restrict__ out = (cx<T>*)a.out + base + (long)tid * es;
   const auto* __restrict__ tw = (const typename TwE<T>::type*)a.tw;
   const LdsCol la{c, TC};
   constexpr int RF0 = Seq<N, V, ORD_FWD>::first(), RFL = Seq<N, V, ORD_FWD>::last();
@@ -252,7 +255,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
   if constexpr (MODE == CM_INV || MODE == CM_INV_VG) {
     if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(tid, r) * es];
+      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RFL>(0, r) * es];
     }
     if constexpr (PRE) twr.load(tid, tw);
     fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
@@ -269,14 +272,14 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
     if (active) {
 #pragma unroll
       for (int r = 0; r < E; ++r) {
-        out[bpos<N, TPL, RF0>(tid, r) * es] = v[r];
+        out[bpos<N, TPL, RF0>(0, r) * es] = v[r];
         if (MODE == CM_INV_VG && (flags & F_NORM)) acc += abs2d(v[r]);
       }
     }
   } else {
     if (active) {
 #pragma unroll
-      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RF0>(tid, r) * es];
+      for (int r = 0; r < E; ++r) v[r] = in[bpos<N, TPL, RF0>(0, r) * es];
     }
     if constexpr (PRE) twr.load(tid, tw);
     if constexpr (MODE == CM_FWD_V) {
@@ -302,7 +305,7 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
     if constexpr (MODE == CM_FWD || MODE == CM_FWD_V) {
       if (active) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) out[bpos<N, TPL, RFL>(tid, r) * es] = v[r];
+        for (int r = 0; r < E; ++r) out[bpos<N, TPL, RFL>(0, r) * es] = v[r];
       }
     } else if constexpr (MODE == CM_KSUM) {
       if (active) {
@@ -332,13 +335,13 @@ __device__ __forceinline__ void col_body(const SweepArgs& a, int bid, int nblk) 
         fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
         if (active) {
 #pragma unroll
-          for (int r = 0; r < E; ++r) out[bpos<N, TPL, RFL>(tid, r) * es] = v[r];
+          for (int r = 0; r < E; ++r) out[bpos<N, TPL, RFL>(0, r) * es] = v[r];
         }
       } else {
         if (active) {
 #pragma unroll
           for (int r = 0; r < E; ++r) {
-            out[bpos<N, TPL, RF0>(tid, r) * es] = v[r];
+            out[bpos<N, TPL, RF0>(0, r) * es] = v[r];
             if (flags & F_NORM) acc += abs2d(v[r]);
           }
         }
