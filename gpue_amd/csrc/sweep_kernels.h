@@ -72,6 +72,10 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
   constexpr int MODE = MODE_ & 15;
   constexpr bool X = (MODE_ & 16) != 0;
   constexpr bool EQ = (MODE_ & 32) != 0;
+  // bit 64: RM_INV / RM_INV_FWD compiled without the gauge branch, for sweeps
+  // whose flags carry no F_GAUGE_POST (its three-transform path sets the
+  // register count of the whole kernel, and so the waves per SIMD)
+  constexpr bool NG = (MODE_ & 64) != 0;
   constexpr int V = RV & 1;
   constexpr bool PRE = (RV >> 1) != 0;
   using P = Plan<N, V>;
@@ -105,7 +109,7 @@ __device__ __forceinline__ void row_body(const SweepArgs& a, int bid, int nblk) 
     if constexpr (PRE) twr.load(tid, tw);
     fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);
     if (flags & F_HALFV_POST) halfV_all<T, E, X, EQ>(p, v, pos0);
-    if (flags & F_GAUGE_POST) {
+    if (!NG && (flags & F_GAUGE_POST)) {
       fft_regs_tw<T, N, V, ORD_FWD, -1, PRE>(v, tid, tw, twr, lds, la, active);
       if (!gauge_rot_lin<T, N, TPL, RFL, E, 0, X, EQ>(p, v, tid, yi, a.invN)) gauge_all<T, E, 0, X, EQ>(p, v, posL, a.invN);
       fft_regs_tw<T, N, V, ORD_REV, +1, PRE>(v, tid, tw, twr, lds, la, active);

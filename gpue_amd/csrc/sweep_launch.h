@@ -73,14 +73,22 @@ template <typename T, int N, int V>
 hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
   if (a.ph.eqk && mode != RM_KSUM) return launch_row_eq<T, N, V>(mode, a, s);
   dim3 grid(row_blocks<N, V>(a.lines)), block(RowCfg<N, V>::WG);
+  // closing sweeps with no gauge operator take the kernels compiled without it
+  const bool ng = !(a.flags & F_GAUGE_POST);
   if constexpr (sizeof(T) == 4) {
     if (a.ph.gstate != 0 && mode != RM_KSUM) {
       switch (mode) {
         case RM_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FWD | 16>), grid, block, 0, s, a); break;
-        case RM_INV: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV | 16>), grid, block, 0, s, a); break;
+        case RM_INV:
+          if (ng) hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV | 16 | 64>), grid, block, 0, s, a);
+          else hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV | 16>), grid, block, 0, s, a);
+          break;
         case RM_GAUGE: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE | 16>), grid, block, 0, s, a); break;
         case RM_FULL: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FULL | 16>), grid, block, 0, s, a); break;
-        case RM_INV_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD | 16>), grid, block, 0, s, a); break;
+        case RM_INV_FWD:
+          if (ng) hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD | 16 | 64>), grid, block, 0, s, a);
+          else hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD | 16>), grid, block, 0, s, a);
+          break;
         case RM_GAUGE_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE_FWD | 16>), grid, block, 0, s, a); break;
         case RM_K: hipLaunchKernelGGL((row_sweep<T, N, V, RM_K | 16>), grid, block, 0, s, a); break;
         default: return hipErrorInvalidValue;
@@ -90,11 +98,17 @@ hipError_t launch_row_t(int mode, const SweepArgs& a, hipStream_t s) {
   }
   switch (mode) {
     case RM_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FWD>), grid, block, 0, s, a); break;
-    case RM_INV: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV>), grid, block, 0, s, a); break;
+    case RM_INV:
+      if (ng) hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV | 64>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV>), grid, block, 0, s, a);
+      break;
     case RM_GAUGE: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE>), grid, block, 0, s, a); break;
     case RM_FULL: hipLaunchKernelGGL((row_sweep<T, N, V, RM_FULL>), grid, block, 0, s, a); break;
     case RM_KSUM: hipLaunchKernelGGL((row_sweep<T, N, V, RM_KSUM>), grid, block, 0, s, a); break;
-    case RM_INV_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD>), grid, block, 0, s, a); break;
+    case RM_INV_FWD:
+      if (ng) hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD | 64>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((row_sweep<T, N, V, RM_INV_FWD>), grid, block, 0, s, a);
+      break;
     case RM_GAUGE_FWD: hipLaunchKernelGGL((row_sweep<T, N, V, RM_GAUGE_FWD>), grid, block, 0, s, a); break;
     case RM_K: hipLaunchKernelGGL((row_sweep<T, N, V, RM_K>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
