@@ -85,29 +85,30 @@ def run_schedule(ctx, imag, real):
 # (each scaled by its reading for a plain 256 MiB device copy: 2.0 and 1.0), per
 # kernel instance and workload; the passes and tables are in profiles/.
 MEASURED_TRAFFIC = {
-    "gpue::col_sweep<double, 512, 5, 2>@512sq": 8806982,
-    "gpue::col_sweep<double, 512, 5, 3>@512sq": 8787582,
-    "gpue::col_sweep<double, 512, 5, 5>@512sq": 8697066,
-    "gpue::col_sweep<double, 512, 5, 6>@512sq": 8937603,
-    "gpue::row_sweep<double, 512, 3, 0>@512sq": 8703468,
-    "gpue::row_sweep<double, 512, 3, 1>@512sq": 8971830,
-    "gpue::row_sweep<double, 512, 3, 2>@512sq": 8783725,
-    "gpue::row_sweep<double, 512, 3, 7>@512sq": 8827460,
-    "gpue::col_sweep<double, 256, 0, 0>@256cube": 536993121,
-    "gpue::col_sweep<double, 256, 0, 1>@256cube": 537006157,
-    "gpue::col_sweep<double, 256, 0, 2>@256cube": 537432365,
-    "gpue::col_sweep<double, 256, 0, 3>@256cube": 537402085,
-    "gpue::col_sweep<double, 256, 0, 5>@256cube": 537292397,
-    "gpue::col_sweep<double, 256, 0, 6>@256cube": 537571773,
-    "gpue::col_sweep<double, 256, 2, 2>@256cube": 537334454,
-    "gpue::col_sweep<float, 256, 0, 0>@256cube": 268572549,
-    "gpue::col_sweep<float, 256, 0, 1>@256cube": 268557373,
-    "gpue::col_sweep<float, 256, 0, 2>@256cube": 268817454,
-    "gpue::row_sweep<double, 256, 1, 0>@256cube": 537077784,
-    "gpue::row_sweep<double, 256, 1, 1>@256cube": 537773393,
-    "gpue::row_sweep<double, 256, 1, 2>@256cube": 538221595,
-    "gpue::row_sweep<float, 256, 0, 0>@256cube": 268647970,
-    "gpue::row_sweep<float, 256, 0, 1>@256cube": 268944772,
+    "gpue::col_sweep<double, 512, 5, 2>@512sq": 8787797,
+    "gpue::col_sweep<double, 512, 5, 3>@512sq": 8784887,
+    "gpue::col_sweep<double, 512, 5, 5>@512sq": 8697118,
+    "gpue::col_sweep<double, 512, 5, 6>@512sq": 8936286,
+    "gpue::row_sweep<double, 512, 3, 0>@512sq": 8717749,
+    "gpue::row_sweep<double, 512, 3, 1>@512sq": 8972530,
+    "gpue::row_sweep<double, 512, 3, 2>@512sq": 8793964,
+    "gpue::row_sweep<double, 512, 3, 7>@512sq": 8841794,
+    "gpue::col_sweep<double, 256, 0, 0>@256cube": 536992027,
+    "gpue::col_sweep<double, 256, 0, 1>@256cube": 536993838,
+    "gpue::col_sweep<double, 256, 0, 2>@256cube": 537408139,
+    "gpue::col_sweep<double, 256, 0, 3>@256cube": 537402176,
+    "gpue::col_sweep<double, 256, 0, 5>@256cube": 537285972,
+    "gpue::col_sweep<double, 256, 0, 6>@256cube": 537569081,
+    "gpue::col_sweep<double, 256, 2, 2>@256cube": 537376379,
+    "gpue::col_sweep<float, 256, 0, 0>@256cube": 268548237,
+    "gpue::col_sweep<float, 256, 0, 1>@256cube": 268545933,
+    "gpue::col_sweep<float, 256, 0, 2>@256cube": 268798608,
+    "gpue::row_sweep<double, 256, 1, 0>@256cube": 537095067,
+    "gpue::row_sweep<double, 256, 1, 1>@256cube": 537334821,
+    "gpue::row_sweep<double, 256, 1, 2>@256cube": 538243034,
+    "gpue::row_sweep<double, 256, 1, 65>@256cube": 537835606,
+    "gpue::row_sweep<float, 256, 0, 0>@256cube": 268648607,
+    "gpue::row_sweep<float, 256, 0, 65>@256cube": 268896874,
 }
 
 RM = {"row_fwd": 0, "row_inv": 1, "row_gauge": 2, "row_full": 3, "row_ksum": 4, "row_inv_fwd": 5,
