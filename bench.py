@@ -119,11 +119,17 @@ CM = {"fwd": 0, "inv": 1, "k": 2, "gauge": 3, "ksum": 4, "fwdv": 5, "invvg": 6, 
 def kernel_symbol(name, prec, n_axis, variant):
     """The template instance rocprofv3 reports for a sweep name (row sweeps
     are instantiated per plan and twiddle loading, variants 0..3 with 4, 5
-    acting as 2, 3; strided sweeps per full variant)."""
+    acting as 2, 3; strided sweeps per full variant). Closing row sweeps
+    without the gauge operator after them ("+gx") run the instance compiled
+    without it (mode bit 64)."""
     t = "double" if prec == 64 else "float"
-    base = name.split("+")[0]
+    parts = name.split("+")
+    base = parts[0]
     if base.startswith("row"):
-        return f"gpue::row_sweep<{t}, {n_axis[0]}, {(variant[0] & 1) + (2 if variant[0] > 1 else 0)}, {RM[base]}>"
+        mode = RM[base]
+        if base in ("row_inv", "row_inv_fwd") and "gx" not in parts[1:]:
+            mode |= 64
+        return f"gpue::row_sweep<{t}, {n_axis[0]}, {(variant[0] & 1) + (2 if variant[0] > 1 else 0)}, {mode}>"
     axis = 1 if base.startswith("coly") else 2
     return f"gpue::col_sweep<{t}, {n_axis[axis]}, {variant[axis]}, {CM[base.split('_', 1)[1]]}>"
 

@@ -26,6 +26,9 @@ def test_more_than_one_gpu_exits_with_a_message():
     ("row_fwd+vpre", 64, (3, 5, 5), "gpue::row_sweep<double, 512, 3, 0>"),
     ("row_gauge_fwd+vpre", 64, (5, 1, 1), "gpue::row_sweep<double, 512, 3, 6>"),   # x: 4, 5 act as 2, 3
     ("row_inv+vpost+gx", 64, (1, 1, 1), "gpue::row_sweep<double, 512, 1, 1>"),
+    # closing sweeps with no gauge operator: the instance compiled without it
+    ("row_inv+vpost+norm", 32, (0, 0, 0), "gpue::row_sweep<float, 512, 0, 65>"),
+    ("row_inv_fwd+vpost+vpre", 64, (3, 5, 5), "gpue::row_sweep<double, 512, 3, 69>"),
     ("row_gauge+norm", 64, (2, 0, 0), "gpue::row_sweep<double, 512, 2, 2>"),
 ])
 def test_kernel_symbols(name, prec, variant, expect):
