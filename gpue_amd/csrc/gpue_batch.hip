@@ -10,17 +10,19 @@
 // [FWD][K][INV][A_y][A_x+norm]) both parities share one launch per position,
 // with per-instance flags, and the extra position is inactive for odd
 // instances; otherwise each parity has its own sequence.  Every imaginary-time
-// sequence ends with one batched norm finalize.  Each group runs on its own
-// stream, with two consecutive steps captured once per gstate as a HIP graph
-// and replayed, so the groups of different shapes (the small ones latency
-// bound) overlap on the device.  Device memory is allocated only in
-// gpue_batch_create.
+// sequence ends with one batched norm finalize.  The groups run on at most
+// four streams (the process's hardware queues; the groups with the most work
+// get one each, the others share the last), with two consecutive steps of a
+// group captured once per gstate as a HIP graph and replayed, so the groups of
+// different shapes (the small ones latency bound) overlap on the device.
+// Device memory is allocated only in gpue_batch_create.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <vector>
 
 #include "ctx_internal.h"
@@ -44,7 +46,7 @@ struct Group {
   gpue_ctx* proto;                  // first member: shape and variants
   std::vector<Sweep> sched[2][2];   // [gstate][parity]
   std::vector<BSeq> seqs[2][2];     // [gstate][phase]
-  hipStream_t stream = nullptr;     // the group's own stream (default priority)
+  hipStream_t stream = nullptr;     // one of the batch's streams (not owned)
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
 };
@@ -56,6 +58,8 @@ bool same_shape(const gpue_ctx* a, const gpue_ctx* b) {
 }
 
 bool same_kind(const Sweep& x, const Sweep& y) { return x.row == y.row && x.axis == y.axis && x.mode == y.mode; }
+
+constexpr size_t kBatchStreams = 4;   // the process's hardware queues
 
 // the shorter list's kinds are a prefix of the longer list's
 bool mergeable(const std::vector<Sweep>& a, const std::vector<Sweep>& b) {
@@ -73,6 +77,9 @@ struct gpue_batch {
   std::vector<int> active;
   std::vector<int> cls;            // parity class: the instance's step parity at phase 0
   std::vector<Group> groups;
+  // at most four streams (default priority), owned here: the groups with the
+  // most work get one each, the others share the last
+  std::vector<hipStream_t> streams;
   int phase = 0;
   // graph mode 1: one graph per pair of steps for the whole batch, captured on
   // `origin` with a branch per shape group (forked to the group streams and
@@ -226,8 +233,8 @@ static void drop_batch_graphs(gpue_batch* b) {
 }
 
 static gpue_status sync_groups(gpue_batch* b) {
-  for (Group& g : b->groups)
-    if (g.stream) BHIP(b, hipStreamSynchronize(g.stream));
+  for (hipStream_t st : b->streams)
+    if (st) BHIP(b, hipStreamSynchronize(st));
   if (b->origin) BHIP(b, hipStreamSynchronize(b->origin));
   return GPUE_OK;
 }
@@ -264,8 +271,8 @@ extern "C" {
 
 void gpue_batch_destroy(gpue_batch* b) {
   if (!b) return;
-  for (Group& g : b->groups)
-    if (g.stream) (void)hipStreamSynchronize(g.stream);
+  for (hipStream_t st : b->streams)
+    if (st) (void)hipStreamSynchronize(st);
   if (b->origin) (void)hipStreamSynchronize(b->origin);
   drop_batch_graphs(b);
   if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
@@ -280,8 +287,8 @@ void gpue_batch_destroy(gpue_batch* b) {
             if (p) (void)hipFree(p);
           if (s.dfin) (void)hipFree(s.dfin);
         }
-  for (Group& g : b->groups)
-    if (g.stream) (void)hipStreamDestroy(g.stream);
+  for (hipStream_t st : b->streams)
+    if (st) (void)hipStreamDestroy(st);
   if (b->terms) (void)hipHostFree(b->terms);
   delete b;
 }
@@ -334,11 +341,27 @@ gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
       b->groups.push_back(g);
     }
   }
-  for (Group& g : b->groups)
-    if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess) {
-      gpue_batch_destroy(b);
-      return GPUE_EHIP;
+  {
+    // A process gets four hardware queues (GPU_MAX_HW_QUEUES); more streams
+    // than that share queues in FIFO order wherever the runtime places them.
+    // So the groups are ranked by work (instances x points), the first three
+    // get a stream each and the rest share the last one, where their graphs
+    // run one after another.
+    std::vector<size_t> order(b->groups.size());
+    std::iota(order.begin(), order.end(), size_t(0));
+    auto work = [&](size_t k) { return (double)b->groups[k].members.size() * (double)b->groups[k].proto->N; };
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work(x) > work(y); });
+    const size_t ns = std::min<size_t>(b->groups.size(), kBatchStreams);
+    for (size_t i = 0; i < ns; ++i) {
+      hipStream_t st = nullptr;
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        gpue_batch_destroy(b);
+        return GPUE_EHIP;
+      }
+      b->streams.push_back(st);
     }
+    for (size_t r = 0; r < order.size(); ++r) b->groups[order[r]].stream = b->streams[std::min(r, ns - 1)];
+  }
   b->join_ev.assign(b->groups.size(), nullptr);
   if (hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess) {
