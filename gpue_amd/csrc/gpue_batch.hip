@@ -80,6 +80,7 @@ struct gpue_batch {
   // at most four streams (default priority), owned here: the groups with the
   // most work get one each, the others share the last
   std::vector<hipStream_t> streams;
+  std::vector<size_t> by_work;     // group indices, most work first (launch order of the replays)
   int phase = 0;
   // graph mode 1: one graph per pair of steps for the whole batch, captured on
   // `origin` with a branch per shape group (forked to the group streams and
@@ -361,6 +362,7 @@ gpue_status gpue_batch_create(gpue_ctx* const* ctxs, int n, gpue_batch** out) {
       b->streams.push_back(st);
     }
     for (size_t r = 0; r < order.size(); ++r) b->groups[order[r]].stream = b->streams[std::min(r, ns - 1)];
+    b->by_work = order;
   }
   b->join_ev.assign(b->groups.size(), nullptr);
   if (hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess ||
@@ -502,9 +504,10 @@ gpue_status gpue_batch_evolve(gpue_batch* b, int nsteps, int gstate) {
       }
       grp.graph[gstate] = g;
     }
-    // pairs of steps, the groups' graphs interleaved across their streams
+    // pairs of steps, the groups' graphs interleaved across their streams, the
+    // group with the most work (the critical path) first
     for (; nsteps - s >= 2; s += 2)
-      for (Group& grp : b->groups) BHIP(b, hipGraphLaunch(grp.gexec[gstate], grp.stream));
+      for (size_t k : b->by_work) BHIP(b, hipGraphLaunch(b->groups[k].gexec[gstate], b->groups[k].stream));
   }
   if (s < nsteps) {
     gpue_status st = launch_phase(b, gstate, 0);
