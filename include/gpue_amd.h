@@ -216,8 +216,10 @@ double      gpue_eqn_eval(const gpue_eqn_node *nodes, int n_nodes, double x, dou
 
 /* --- batched instances (no reference counterpart: GPUE solves one problem per
  * process; its bin/run.sh starts several processes).  Instances of one shape
- * share every sweep launch; each shape group runs on its own stream, two steps
- * captured as a HIP graph and replayed, so the groups overlap on the device.
+ * share every sweep launch; the shape groups run on at most four streams (the
+ * device's hardware queues per process: the three groups with the most work
+ * get one each, the rest share the fourth), two steps of a group captured as
+ * a HIP graph and replayed, so the groups overlap on the device.
  * The kernel variants are the contexts' own (equal within a group; a batch
  * may prefer other variants than a lone instance: the groups share the
  * device).  Contexts stay owned by the caller and must outlive the
@@ -235,7 +237,7 @@ gpue_status gpue_batch_evolve(gpue_batch *b, int nsteps, int gstate);
  * waited for once: the values are those gpue_energy returns per context. */
 gpue_status gpue_batch_energy(gpue_batch *b, double *energy, double *mu);
 /* How pairs of steps are replayed: 0 (default) one graph per shape group on the
- * group's stream, launched group by group; 1 one graph for the whole batch, a
+ * group's stream, launched group by group, most work first; 1 one graph for the whole batch, a
  * branch per group forked onto the group streams, so a pair of steps of every
  * instance is one launch; 2 no graphs (every sweep launched on its own, for
  * kernel tracers). */
