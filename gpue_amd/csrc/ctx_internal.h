@@ -57,6 +57,11 @@ struct gpue_ctx {
   int graph_steps = 8;
   std::vector<gpue::StepGraph> graphs;
   hipEvent_t gev = nullptr;    // recorded after every graph replay (a shared stream is not waited for)
+  // the energy evaluation of gpue_batch_energy as one graph (energy_launch_graph),
+  // with the host buffer its terms are copied to; dropped with the step graphs
+  hipGraph_t egraph = nullptr;
+  hipGraphExec_t egexec = nullptr;
+  double* egraph_dst = nullptr;
   long graph_clock = 0;        // use counter for the least-recently-used eviction
   double* partials = nullptr;
   size_t partials_n = 0;
@@ -116,6 +121,7 @@ gpue_status launch(gpue_ctx* c, const Sweep& s, const Phys& ph, const void* in, 
 gpue_status apply_pending(gpue_ctx* c);
 gpue_status evolve_async(gpue_ctx* c, int first_step, int nsteps, int gstate);
 gpue_status energy_launch(gpue_ctx* c, double* host_dst);
+gpue_status energy_launch_graph(gpue_ctx* c, double* host_dst);
 // psi to the host, waiting on `ev` (recorded after the copy) rather than the stream.
 gpue_status download_wfc_ev(gpue_ctx* c, double* interleaved, hipEvent_t ev);
 // A context on a given stream (null: its own), and new parameters for one of the

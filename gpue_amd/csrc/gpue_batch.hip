@@ -638,7 +638,7 @@ gpue_status gpue_batch_energy(gpue_batch* b, double* energy, double* mu) {
   if (!b || !energy || !mu) return GPUE_EINVAL;
   const size_t n = b->ctx.size();
   for (size_t i = 0; i < n; ++i) {   // queue every instance's evaluation first
-    gpue_status st = energy_launch(b->ctx[i], b->terms + 6 * i);
+    gpue_status st = energy_launch_graph(b->ctx[i], b->terms + 6 * i);
     if (st != GPUE_OK) return bfail(b, st, b->ctx[i]->err);
   }
   gpue_status out = GPUE_OK;

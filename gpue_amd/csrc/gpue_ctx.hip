@@ -372,7 +372,18 @@ static bool valid_dim(int n) { return sweep_supported(n); }
 // Any change of fields, parameters or kernel variants invalidates the captured
 // graphs.  Replays may still be queued (evolve returns without waiting in real
 // time), so the stream is drained before an executable graph is destroyed.
+static void drop_energy_graph(gpue_ctx* c) {
+  if (!c->egexec) return;
+  (void)hipStreamSynchronize(c->stream);   // a replay may be queued (the context's own stream)
+  (void)hipGraphExecDestroy(c->egexec);
+  if (c->egraph) (void)hipGraphDestroy(c->egraph);
+  c->egexec = nullptr;
+  c->egraph = nullptr;
+  c->egraph_dst = nullptr;
+}
+
 static void drop_graphs(gpue_ctx* c) {
+  drop_energy_graph(c);
   if (c->graphs.empty()) return;
   if (c->gev) (void)hipEventSynchronize(c->gev);   // the last replay (graph launches record it)
   else if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -1017,6 +1028,40 @@ gpue_status gpue::energy_launch(gpue_ctx* c, double* host_dst) {
   if (e == hipSuccess && host_dst)
     e = hipMemcpyAsync(host_dst, c->results, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e != hipSuccess) return fail(c, GPUE_EHIP, hipGetErrorString(e));
+  return GPUE_OK;
+}
+
+// energy_launch as one graph, captured at the first call and replayed after
+// (about eight launches become one; gpue_batch_energy evaluates every instance
+// at each convergence check). Taken only where the launches depend on no host
+// state: a separable K, static fields, no kernel timing, and the context's own
+// stream (a capture there sees no other caller's work); drop_graphs discards
+// the graph whenever parameters, fields or variants change.
+gpue_status gpue::energy_launch_graph(gpue_ctx* c, double* host_dst) {
+  if (!host_dst || c->t_dep || c->timing || !c->own_stream) return energy_launch(c, host_dst);
+  Phys ph;
+  gpue_status st = energy_phys(c, &ph, false);
+  if (st != GPUE_OK) return st;
+  if (ph.k_kind != K_SEP) return energy_launch(c, host_dst);
+  if (c->egexec && c->egraph_dst != host_dst) drop_energy_graph(c);
+  if (!c->egexec) {
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    st = energy_launch(c, host_dst);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (st != GPUE_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return st != GPUE_OK ? st : fail(c, GPUE_EHIP, std::string("energy graph capture: ") + hipGetErrorString(e));
+    }
+    if (hipGraphInstantiate(&c->egexec, g, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      c->egexec = nullptr;
+      return fail(c, GPUE_EHIP, "energy graph instantiate");
+    }
+    c->egraph = g;
+    c->egraph_dst = host_dst;
+  }
+  HIPCHK(c, hipGraphLaunch(c->egexec, c->stream));
   return GPUE_OK;
 }
 
