@@ -234,7 +234,10 @@ gpue_status gpue_batch_set_active(gpue_batch *b, int index, int active);
 gpue_status gpue_batch_evolve(gpue_batch *b, int nsteps, int gstate);
 /* Energy and chemical potential of every instance (arrays of n, in the order
  * of gpue_batch_create), evaluated on all instances' streams at once and
- * waited for once: the values are those gpue_energy returns per context. */
+ * waited for once: the values are those gpue_energy returns per context.
+ * An instance with a separable K and static fields on its own stream has its
+ * evaluation captured as one HIP graph at the first call and replayed after;
+ * any change of its parameters, fields or variants drops the graph. */
 gpue_status gpue_batch_energy(gpue_batch *b, double *energy, double *mu);
 /* How pairs of steps are replayed: 0 (default) one graph per shape group on the
  * group's stream, launched group by group, most work first; 1 one graph for the whole batch, a
