@@ -1034,11 +1034,11 @@ gpue_status gpue::energy_launch(gpue_ctx* c, double* host_dst) {
 // energy_launch as one graph, captured at the first call and replayed after
 // (about eight launches become one; gpue_batch_energy evaluates every instance
 // at each convergence check). Taken only where the launches depend on no host
-// state: a separable K, static fields, no kernel timing, and the context's own
+// state: a separable K, static fields, no ramp, no kernel timing, and the context's own
 // stream (a capture there sees no other caller's work); drop_graphs discards
 // the graph whenever parameters, fields or variants change.
 gpue_status gpue::energy_launch_graph(gpue_ctx* c, double* host_dst) {
-  if (!host_dst || c->t_dep || c->timing || !c->own_stream) return energy_launch(c, host_dst);
+  if (!host_dst || c->t_dep || c->ramp_type || c->timing || !c->own_stream) return energy_launch(c, host_dst);
   Phys ph;
   gpue_status st = energy_phys(c, &ph, false);
   if (st != GPUE_OK) return st;
